@@ -1,0 +1,190 @@
+/*
+ * sph_hip.h - C ABI of libsph_hip.so, the MI355X (gfx950) SPH step engine.
+ *
+ * The library replaces libSPH.so of tauroid/opencl-sph for the simulation
+ * step.  Host-side functions (config, psdata) keep the reference's names,
+ * signatures and struct layout so that ctypes bindings written for libSPH.so
+ * bind to them unchanged.  Device-side functions replace the OpenCL runtime
+ * (src/opencl/particle_system_host.h) with a HIP runtime: the by-value
+ * `psdata_opencl` struct becomes an opaque `psdata_hip *` handle, and every
+ * device call returns an int status (0 = success) instead of aborting; the
+ * text of the last error is available from sph_hip_last_error().
+ *
+ * Citations are file:line in the reference repository.
+ */
+#ifndef SPH_HIP_H_
+#define SPH_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * psdata container.  Layout identical to src/datatypes.h:4-25 and
+ * src/particle_system.h:18-48 (the Python layer reads these members directly).
+ * ------------------------------------------------------------------------- */
+typedef enum {
+    T_CHAR,
+    T_UINT,
+    T_INT,
+    T_FLOAT32,
+    T_FLOAT64,
+    TYPE_COUNT
+} FieldType;
+
+typedef struct {
+    FieldType type;
+    union {
+        char     *c;
+        uint32_t *u;
+        int32_t  *i;
+        float    *f4;
+        double   *f8;
+        void     *ptr;
+    };
+} DataPtr;
+
+typedef struct {
+    unsigned int   num_fields;
+    const char    *names;
+    unsigned int  *names_offsets;
+    unsigned int  *dimensions;
+    unsigned int  *num_dimensions;
+    unsigned int  *dimensions_offsets;
+    unsigned int  *entry_sizes;
+
+    void          *data;
+    unsigned int  *data_sizes;
+    unsigned int  *data_offsets;
+
+    DataPtr       *data_ptrs;
+
+    unsigned int   num_host_fields;
+    char         **host_names;
+    void         **host_data;
+    unsigned int  *host_data_size;
+} psdata;
+
+/* ---------------------------------------------------------------------------
+ * Logging.  Replaces src/note.h (set_log_level, note).
+ * ------------------------------------------------------------------------- */
+void set_log_level(unsigned int level);                       /* src/note.c:9-11  */
+void note(unsigned int level, const char *format, ...);        /* src/note.c:13-33 */
+
+/* ---------------------------------------------------------------------------
+ * Config loader.  Replaces src/config.h:4-6.  load_config returns 1 when the
+ * file cannot be read (src/config.c:49-52), 0 otherwise.
+ * ------------------------------------------------------------------------- */
+int         load_config(const char *relative_path);           /* src/config.c:19-91   */
+const char *get_config_section(const char *heading);          /* src/config.c:92-98   */
+void        unload_config(void);                               /* src/config.c:99-106  */
+
+/* ---------------------------------------------------------------------------
+ * psdata builder and accessors.  Replaces src/build_psdata.h:6-7 and
+ * src/particle_system.h:56-77.  The blob layout (field order, type sort,
+ * alignment, auto-added grid arrays) is byte-identical to the reference.
+ * ------------------------------------------------------------------------- */
+void build_psdata_from_string(psdata *data, const char *string);   /* src/build_psdata.c:585-613 */
+void build_psdata(psdata *data, const char *path);                   /* src/build_psdata.c:614-636 */
+int  get_field_psdata(psdata data, const char *name);                /* src/particle_system.c:84-91 */
+void set_field_psdata(psdata *data, const char *name, void *field,
+                      unsigned int size, unsigned int offset);       /* src/particle_system.c:93-97 */
+unsigned int psdata_names_size(psdata data);                         /* src/particle_system.c:99-103 */
+unsigned int psdata_dimensions_size(psdata data);                    /* src/particle_system.c:105-108 */
+unsigned int psdata_data_size(psdata data);                          /* src/particle_system.c:110-112 */
+int  create_host_field_psdata(psdata *data, const char *name,
+                              void *field, unsigned int size);       /* src/particle_system.c:116-155 */
+int  get_host_field_psdata(psdata *data, const char *name);          /* src/particle_system.c:157-164 */
+void display_psdata(psdata data, const char *const *mask);           /* src/particle_system.c:47-82 */
+void free_psdata(psdata *data);                                      /* src/particle_system.c:218-255 */
+
+/* Status of the last build_psdata* call on this thread: 0 = ok.  The
+ * reference returns void and asserts; this lets callers detect failure. */
+int  psdata_last_build_status(void);
+
+/* ---------------------------------------------------------------------------
+ * Device runtime (HIP).  Replaces src/opencl/particle_system_host.h:55-80.
+ * ------------------------------------------------------------------------- */
+typedef struct psdata_hip psdata_hip;   /* opaque; replaces psdata_opencl (:7-47) */
+
+const char *sph_hip_last_error(void);
+
+/* init_opencl (:55): select the HIP device (the first one, or $SPH_HIP_DEVICE)
+ * and create the stream every handle launches on. */
+int init_hip(void);
+/* terminate_opencl (:60) */
+int terminate_hip(void);
+
+/* create_psdata_opencl (:56): mirror `data` on the device.  `file_list` is the
+ * $opencl_kernel_files section ("common, fluids", "common, solids,
+ * solid_corotation, constraints", ...); it selects which named kernels the
+ * handle accepts, exactly as the list of .cl files does in the reference. */
+int create_psdata_hip(psdata *data, const char *file_list, psdata_hip **out);
+/* free_psdata_opencl (:59) */
+int free_psdata_hip(psdata_hip *pso);
+
+/* sync_psdata_* (:62-65).  Device state is kept in cell-sorted order; these
+ * calls present and accept it in psdata (particle-index) order. */
+int sync_psdata_host_to_device_hip(psdata data, psdata_hip *pso, int full);
+int sync_psdata_device_to_host_hip(psdata data, psdata_hip *pso);
+int sync_psdata_fields_host_to_device_hip(psdata data, psdata_hip *pso,
+                                          size_t num_fields,
+                                          const char *const *field_names);
+int sync_psdata_fields_device_to_host_hip(psdata data, psdata_hip *pso,
+                                          size_t num_fields,
+                                          const char *const *field_names);
+
+/* call_for_all_particles_device_opencl (:74): run the named kernel over all
+ * particles.  Accepted names: compute_density, compute_forces_fluids,
+ * step_forward, init_original_position, compute_original_density,
+ * compute_rotations_and_strains, compute_deformations, compute_stresses,
+ * compute_forces_solids, apply_plane_constraints, find_particle_bins. */
+int call_for_all_particles_device_hip(psdata_hip *pso, const char *kernel_name);
+
+/* compute_particle_bins_device_opencl (:76): key, count, scan, stable insert
+ * (ascending particle index within a cell) and cell-sorted reorder. */
+int compute_particle_bins_device_hip(psdata_hip *pso);
+
+/* compute_density/forces/step_forward_device_opencl (:78-80) */
+int compute_density_device_hip(psdata_hip *pso);
+int compute_forces_device_hip(psdata_hip *pso);
+int step_forward_device_hip(psdata_hip *pso);
+
+/* No counterpart in the reference: run `num_steps` whole simulation steps of
+ * `system` ("fluid", "corotational" or "svk"), the sequences of
+ * src/mex/simstep_fluid.c:18-23 and scripts/sph/systems.py:148-189, without
+ * host round trips.  When `apply_constraints` is non-zero,
+ * apply_plane_constraints runs after step_forward as in
+ * scripts/straintest.py:224. */
+int step_device_hip(psdata_hip *pso, const char *system, unsigned int num_steps,
+                    int apply_constraints);
+
+/* Block until all work queued on the handle's stream has finished. */
+int synchronize_hip(psdata_hip *pso);
+
+/* Precision of neighbour sums: 0 = plain FP32 in reference order (default),
+ * 1 = Kahan-compensated FP32.  polar_fp64 != 0 computes the corotational
+ * polar decomposition and the SVK inverse in FP64 (default 1). */
+int set_precision_hip(psdata_hip *pso, int compensated_sums, int polar_fp64);
+
+/* Per-kernel timing with HIP events on the handle's stream (the reference
+ * creates a profiling queue at particle_system_host.c:92-93 but never reads
+ * it).  get_kernel_stats_hip fills up to `capacity` entries and returns the
+ * number of distinct kernels timed (or -1 on error). */
+int set_profiling_hip(psdata_hip *pso, int enabled);
+int reset_kernel_stats_hip(psdata_hip *pso);
+int get_kernel_stats_hip(psdata_hip *pso, int capacity, const char **names,
+                         unsigned long long *launches, double *total_ms);
+
+/* Device-side reductions in FP64 over the active particles: total momentum
+ * (3), total kinetic energy, and the centre of mass (3).  out has 7 doubles. */
+int global_sums_hip(psdata_hip *pso, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_HIP_H_ */

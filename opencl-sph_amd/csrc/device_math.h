@@ -1,0 +1,208 @@
+// Device-side math for the SPH passes: float3 helpers, the smoothing kernels,
+// symmetric 3x3 storage and the fixed-sweep Jacobi eigen solver.
+//
+// Every expression keeps the operand order of the reference kernels
+// (kernels/common.cl:78-350) and the library is compiled without FMA
+// contraction, so each result is the IEEE-rounded value of the same sequence
+// of operations the CPU oracle performs.
+//
+// Symmetric 3x3 matrices use the order [00, 11, 22, 01, 02, 12]
+// (kernels/common.cl:27-35); full 3x3 matrices are row-major.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#pragma clang fp contract(off)
+
+namespace sphk {
+
+struct V3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 load3(const float *p, unsigned i) {
+    return V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+}
+__device__ __forceinline__ void store3(float *p, unsigned i, V3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return V3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float length(V3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// Reflection of point `a` in the plane through `pt` with normal `nrm`:
+// a - 2 n (n . (a - pt))   (kernels/common.cl:672).
+__device__ __forceinline__ V3 reflect_point(V3 a, V3 pt, V3 nrm) {
+    float d = dot(nrm, a - pt);
+    return a - (2.0f * nrm) * d;
+}
+// Reflection of a direction: a - 2 n (n . a)   (kernels/solid_corotation.cl:358).
+__device__ __forceinline__ V3 reflect_dir(V3 a, V3 nrm) {
+    float d = dot(nrm, a);
+    return a - (2.0f * nrm) * d;
+}
+
+// Smoothing kernels (Mueller et al. 2003) with host-precomputed constants
+//   poly6 = 315/(64 pi h^9), spiky = -45/(pi h^6), lap = 45/(pi h^6).
+struct KernelConsts {
+    float h, h2, poly6, spiky, lap;
+};
+
+// W_poly6(r), zero beyond h   (kernels/common.cl:78-80)
+__device__ __forceinline__ float w_poly6(float r, const KernelConsts &k) {
+    if (r > k.h) return 0.0f;
+    float t = k.h2 - r * r;
+    return k.poly6 * (t * t * t);
+}
+// grad W_spiky(v)   (kernels/common.cl:81-87)
+__device__ __forceinline__ V3 grad_spiky(V3 v, const KernelConsts &k) {
+    float r = length(v);
+    if (r > k.h) return V3{0.0f, 0.0f, 0.0f};
+    V3 dir = v / r;
+    float s = (k.h - r) * (k.h - r);
+    return (dir * k.spiky) * s;
+}
+// lap W_visc(r)   (kernels/common.cl:88-90)
+__device__ __forceinline__ float lap_visc(float r, const KernelConsts &k) {
+    if (r > k.h) return 0.0f;
+    return k.lap * (k.h - r);
+}
+
+// ---- 3x3 algebra ---------------------------------------------------------
+enum { S00 = 0, S11 = 1, S22 = 2, S01 = 3, S02 = 4, S12 = 5 };
+
+__device__ __forceinline__ int sym_index(int r, int c) {
+    // [00,11,22,01,02,12]
+    return r == c ? r : (r + c == 1 ? S01 : (r + c == 2 ? S02 : S12));
+}
+
+// s * v with s symmetric   (kernels/common.cl:230-238)
+__device__ __forceinline__ V3 sym_mul(const float *s, V3 v) {
+    return V3{s[S00] * v.x + s[S01] * v.y + s[S02] * v.z,
+              s[S01] * v.x + s[S11] * v.y + s[S12] * v.z,
+              s[S02] * v.x + s[S12] * v.y + s[S22] * v.z};
+}
+// m * v, m row-major   (kernels/common.cl:221-229)
+__device__ __forceinline__ V3 mat_mul(const float *m, V3 v) {
+    return V3{m[0] * v.x + m[1] * v.y + m[2] * v.z,
+              m[3] * v.x + m[4] * v.y + m[5] * v.z,
+              m[6] * v.x + m[7] * v.y + m[8] * v.z};
+}
+
+// out = a * s, s symmetric   (kernels/common.cl:187-199)
+template <typename T>
+__device__ __forceinline__ void mul_full_sym(const T *a, const T *s, T *out) {
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            out[3 * r + c] = a[3 * r] * s[sym_index(0, c)] + a[3 * r + 1] * s[sym_index(1, c)] +
+                             a[3 * r + 2] * s[sym_index(2, c)];
+}
+// out = a * b, both full   (kernels/common.cl:208-220)
+template <typename T>
+__device__ __forceinline__ void mul_full(const T *a, const T *b, T *out) {
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            out[3 * r + c] = a[3 * r] * b[c] + a[3 * r + 1] * b[3 + c] + a[3 * r + 2] * b[6 + c];
+}
+// P diag(d) P^T, symmetric result   (kernels/common.cl:169-186)
+template <typename T>
+__device__ __forceinline__ void p_d_pt(const T *d, const T *p, T *out) {
+    T d0p00 = d[0] * p[0], d1p01 = d[1] * p[1], d2p02 = d[2] * p[2];
+    T d0p10 = d[0] * p[3], d1p11 = d[1] * p[4], d2p12 = d[2] * p[5];
+    T d0p20 = d[0] * p[6], d1p21 = d[1] * p[7], d2p22 = d[2] * p[8];
+    out[S00] = p[0] * d0p00 + p[1] * d1p01 + p[2] * d2p02;
+    out[S01] = p[0] * d0p10 + p[1] * d1p11 + p[2] * d2p12;
+    out[S02] = p[0] * d0p20 + p[1] * d1p21 + p[2] * d2p22;
+    out[S11] = p[3] * d0p10 + p[4] * d1p11 + p[5] * d2p12;
+    out[S12] = p[3] * d0p20 + p[4] * d1p21 + p[5] * d2p22;
+    out[S22] = p[6] * d0p20 + p[7] * d1p21 + p[8] * d2p22;
+}
+template <typename T>
+__device__ __forceinline__ T det3(const T *m) {
+    return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+           m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+__device__ __forceinline__ float t_atan(float x) { return atanf(x); }
+__device__ __forceinline__ double t_atan(double x) { return atan(x); }
+__device__ __forceinline__ float t_sin(float x) { return sinf(x); }
+__device__ __forceinline__ double t_sin(double x) { return sin(x); }
+__device__ __forceinline__ float t_cos(float x) { return cosf(x); }
+__device__ __forceinline__ double t_cos(double x) { return cos(x); }
+__device__ __forceinline__ float t_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double t_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float t_abs(float x) { return fabsf(x); }
+__device__ __forceinline__ double t_abs(double x) { return fabs(x); }
+
+// One Givens rotation that zeroes off-diagonal `od` (3, 4 or 5) of the
+// symmetric `a`, applied to the eigenvector columns of `p`
+// (kernels/common.cl:95-155).  angle = 1/2 atan(2 a_xy / (a_xx - a_yy)); when
+// a_xy is exactly 0 the rotation is the identity (the reference evaluates
+// 0/0 there when a_xx == a_yy).
+template <typename T>
+__device__ __forceinline__ void givens(const T *a, const T *p, int od, T *ao, T *po) {
+    const int k = od - 3;
+    const int ixx = k == 2 ? 1 : 0;
+    const int iyy = k == 0 ? 1 : 2;
+    const int iee = 2 - k;
+    const int iex = k == 0 ? 4 : 3;
+    const int iey = k == 2 ? 4 : 5;
+    T angle = T(0);
+    if (a[od] != T(0)) angle = T(0.5) * t_atan(T(2) * a[od] / (a[ixx] - a[iyy]));
+    const T c = t_cos(angle), s = t_sin(angle);
+    const T c2 = c * c, s2 = s * s, cs = c * s;
+    ao[ixx] = a[ixx] * c2 + a[iyy] * s2 + T(2) * a[od] * cs;
+    ao[od] = -a[ixx] * cs + a[iyy] * cs + a[od] * (c2 - s2);
+    ao[iyy] = a[ixx] * s2 + a[iyy] * c2 - T(2) * a[od] * cs;
+    ao[iee] = a[iee];
+    ao[iex] = a[iex] * c + a[iey] * s;
+    ao[iey] = -a[iex] * s + a[iey] * c;
+    for (int r = 0; r < 3; ++r) {
+        po[3 * r + ixx] = p[3 * r + ixx] * c + p[3 * r + iyy] * s;
+        po[3 * r + iyy] = -p[3 * r + ixx] * s + p[3 * r + iyy] * c;
+        po[3 * r + iee] = p[3 * r + iee];
+    }
+}
+
+// Index (3..5) of the largest |off-diagonal|, first one on ties, 3 if all 0
+// (kernels/common.cl:156-168).
+template <typename T>
+__device__ __forceinline__ int max_offdiag(const T *a) {
+    T m = T(0);
+    int im = 0;
+    for (int q = 0; q < 3; ++q) {
+        T v = t_abs(a[3 + q]);
+        if (v > m) { m = v; im = q; }
+    }
+    return im + 3;
+}
+
+// Eight max-pivot rotations (kernels/common.cl:257-295 with iterations = 8,
+// threshold = 0): eigenvalues in ev, eigenvectors in the columns of p.
+template <typename T>
+__device__ __forceinline__ void jacobi8(const T *a_in, T *ev, T *p) {
+    T a[6], b[6], q[9];
+    for (int t = 0; t < 6; ++t) a[t] = a_in[t];
+    for (int t = 0; t < 9; ++t) p[t] = (t % 4 == 0) ? T(1) : T(0);
+    int od = max_offdiag(a);
+#pragma unroll
+    for (int it = 0; it < 8; it += 2) {
+        givens(a, p, od, b, q);
+        od = max_offdiag(b);
+        givens(b, q, od, a, p);
+        od = max_offdiag(a);
+    }
+    ev[0] = a[0];
+    ev[1] = a[1];
+    ev[2] = a[2];
+}
+
+}  // namespace sphk

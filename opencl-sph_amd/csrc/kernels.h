@@ -1,0 +1,113 @@
+// Launch interface between the HIP runtime (runtime.hip) and the device
+// kernels (kernels_*.hip).  Internal to libsph_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sphk {
+
+constexpr int kMaxSymmetryPlanes = 5;      // 2^5 plane combinations fit one mask word
+constexpr int kMaxPlaneConstraints = 32;   // bit c of plane_constraints_particles
+
+// Uniform grid: res and the box [lo, hi] per axis.
+struct GridGeom {
+    unsigned rx, ry, rz, layer, ncells;
+    float lo[3], hi[3];
+};
+
+// Everything the per-particle passes read besides the per-particle fields.
+struct StepParams {
+    unsigned n;               // active particles (slots [0, n))
+    GridGeom grid;
+    float h, h2, poly6, spiky, lap;
+    float mass, dt;
+    float restdens, stiffness, viscosity;
+    float gravity[3];
+    float youngs, poisson;
+    unsigned nplanes;                              // symmetry planes (0: no field)
+    unsigned active_planes;                        // bit s: plane s has a non-zero normal
+    float planes[kMaxSymmetryPlanes][6];           // point xyz, normal xyz
+    unsigned nconstraints;                         // plane constraints
+    float constraints[kMaxPlaneConstraints][6];
+    unsigned body_count;                           // 0: no body fields
+    int eos_tait;                                  // conf field eos_tait: Tait EOS (exponent 7)
+    int mass_consistent;                           // conf field mass_consistent: fluid force times m
+    int compensated;                               // Kahan-compensated neighbour sums
+    int polar_fp64;                                // FP64 polar decomposition / inverse
+};
+
+// Device pointers of the per-particle fields in the current slot order.
+// Optional fields are nullptr when the conf does not define them.
+struct FieldPtrs {
+    float *position, *velocity, *force, *acceleration;
+    float *density, *density0, *originalpos;
+    float *rotation, *strain, *stress, *deformation;
+    float *apq_temp, *intermediate;
+    float *force_elastic, *force_viscous, *cutoff_force;
+    unsigned *num_neighbours, *body_number, *plane_constraints_particles;
+    const unsigned *body_dynamic;                  // [body_count], not per particle
+    unsigned *gridcell;                            // per-slot cell key
+    const unsigned *celloffset, *gridcount;        // per cell
+};
+
+// ---- neighbour grid (kernels_grid.hip) ------------------------------------
+struct BinScratch {
+    unsigned *key;        // [pnum] cell of each old slot
+    unsigned *local;      // [pnum] arrival order inside the cell
+    unsigned *tmp_id;     // [pnum] particle id at each unsorted cell position
+    unsigned *tmp_src;    // [pnum] old slot at each unsorted cell position
+    unsigned *tmp_key;    // [pnum] cell at each unsorted cell position
+    unsigned *tile_sums;  // [ceil(ncells / kScanTile) + 1]
+};
+constexpr unsigned kScanTile = 4096;
+
+hipError_t launch_fill_u32(unsigned *p, size_t count, unsigned value, hipStream_t s);
+hipError_t launch_iota_u32(unsigned *p, size_t count, hipStream_t s);
+hipError_t launch_cell_keys(const float *position, unsigned n, const GridGeom &g, unsigned *key_out,
+                            hipStream_t s);
+hipError_t launch_key_count(const float *position, unsigned n, const GridGeom &g, BinScratch &b,
+                            unsigned *gridcount, hipStream_t s);
+hipError_t launch_exclusive_scan(const unsigned *in, unsigned count, unsigned *out,
+                                 unsigned *tile_sums, hipStream_t s);
+hipError_t launch_scatter_unsorted(unsigned n, const BinScratch &b, const unsigned *celloffset,
+                                   const unsigned *slot_id, hipStream_t s);
+// Stable placement: ascending particle id inside every cell.  Writes the
+// gather map (perm[new] = old slot), new slot ids, per-slot cell keys and the
+// cellparticles array, and gathers up to two 3-float fields on the way.
+hipError_t launch_rank_place(unsigned n, const BinScratch &b, const unsigned *celloffset,
+                             const unsigned *gridcount, unsigned *perm, unsigned *slot_id_new,
+                             unsigned *gridcell, unsigned *cellparticles, const float *vec_src0,
+                             float *vec_dst0, const float *vec_src1, float *vec_dst1,
+                             hipStream_t s);
+
+// Field movement between orders.  `words` is the number of 32-bit words per
+// particle.  gather: dst[f] = src[map[f]] for f < n, dst[f] = src[f] for
+// n <= f < count.  scatter: dst[map[f]] = src[f] likewise.
+hipError_t launch_gather_words(const void *src, void *dst, const unsigned *map, unsigned n,
+                               unsigned count, unsigned words, hipStream_t s);
+hipError_t launch_scatter_words(const void *src, void *dst, const unsigned *map, unsigned n,
+                                unsigned count, unsigned words, hipStream_t s);
+
+// ---- per-particle passes (kernels_fluid.hip, kernels_solid.hip) ------------
+hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_compute_original_density(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_forces_fluids(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_step_forward(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_plane_constraints(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_init_original_position(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_rotations_and_strains(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_stresses_corotational(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_deformations_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_stresses_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_forces_solids(const StepParams &p, const FieldPtrs &f, bool corotational,
+                                hipStream_t s);
+
+// FP64 reductions over slots [0, n): momentum (3), kinetic energy, centre of
+// mass (3).  `partials` holds 7 * kReduceBlocks doubles.
+constexpr unsigned kReduceBlocks = 256;
+hipError_t launch_global_sums(const StepParams &p, const FieldPtrs &f, double *partials,
+                              double *out7, hipStream_t s);
+
+}  // namespace sphk

@@ -1,0 +1,283 @@
+// Density, fluid forces, integrator, plane constraints and the solid setup
+// passes that share the density loop.
+//
+// Reference kernels: compute_density (kernels/common.cl:610-690),
+// compute_forces_fluids (kernels/fluids.cl:1-63), step_forward
+// (kernels/common.cl:691-722), apply_plane_constraints
+// (kernels/constraints.cl:1-30), init_original_position and
+// compute_original_density (kernels/solids.cl:21-107).
+//
+// One thread per active slot; every sum runs over the candidates in the
+// reference's visiting order, so with compensated sums off each output is
+// the same IEEE operation sequence as the sequential oracle.
+#include "neighbours.h"
+
+#pragma clang fp contract(off)
+
+namespace sphk {
+namespace {
+
+constexpr unsigned kBlock = 256;
+
+template <bool KAHAN>
+struct Acc {
+    float s = 0.0f;
+    __device__ __forceinline__ void add(float v) { s = s + v; }
+    __device__ __forceinline__ float value() const { return s; }
+};
+template <>
+struct Acc<true> {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float v) {
+        float y = v - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ float value() const { return s; }
+};
+
+// ORIGINAL = compute_original_density (density0, no neighbour count).
+template <bool ORIGINAL, bool KAHAN>
+__global__ void __launch_bounds__(kBlock) k_density(StepParams p, FieldPtrs f) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const KernelConsts kc = kconsts(p);
+    const float *__restrict__ pos = f.position;
+    const V3 xi = load3(pos, i);
+    const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
+    const unsigned ncombo = 1u << p.nplanes;
+    Acc<KAHAN> rho;
+    unsigned nn = 0;
+    for_each_candidate(p.grid, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+        const V3 xj = load3(pos, j);
+        const float r = length(xj - xi);
+        if (!(r > kc.h)) rho.add(w_poly6(r, kc));
+        if (!ORIGINAL && r < kc.h) ++nn;
+        if (inear) {
+            const unsigned ok = planes_near_j(p, xj, inear);
+            if (ok) {
+                for (unsigned b = 1; b < ncombo; ++b) {
+                    if (b & ~ok) continue;
+                    const float rr = length(reflect_combo(p, b, xj) - xi);
+                    if (rr < kc.h) {
+                        rho.add(w_poly6(rr, kc));
+                        if (!ORIGINAL) ++nn;
+                    }
+                }
+            }
+        }
+    });
+    const float out = rho.value() * p.mass;
+    if (ORIGINAL) {
+        f.density0[i] = out;
+    } else {
+        f.density[i] = out;
+        if (f.num_neighbours) f.num_neighbours[i] = nn;
+    }
+}
+
+// Equation of state: the reference's linear p = k (rho - rho0)
+// (kernels/fluids.cl:28-29), or Tait with exponent 7 when the conf sets
+// eos_tait.
+__device__ __forceinline__ float pressure(const StepParams &p, float rho) {
+    if (!p.eos_tait) return (rho - p.restdens) * p.stiffness;
+    const float r = rho / p.restdens;
+    const float r2 = r * r, r4 = r2 * r2;
+    return p.stiffness * (r4 * r2 * r - 1.0f);
+}
+
+// Penalty walls at +-2 m, evaluated in double as in the reference macro
+// (kernels/common.cl:37-43 with the double literals of kernels/fluids.cl:60).
+__device__ __forceinline__ float cube_wall(float x, float fc) {
+    const double lo = -2.0, hi = 2.0;
+    const double xd = static_cast<double>(x);
+    if (xd < lo + static_cast<double>(0.2f))
+        return static_cast<float>(static_cast<double>(fc) + (1.0 / (-lo + xd) - 5.0));
+    if (xd > hi - static_cast<double>(0.2f))
+        return static_cast<float>(static_cast<double>(fc) - (1.0 / (hi - xd) - 5.0));
+    return fc;
+}
+
+template <bool KAHAN>
+__global__ void __launch_bounds__(kBlock) k_forces_fluids(StepParams p, FieldPtrs f) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const KernelConsts kc = kconsts(p);
+    const float *__restrict__ pos = f.position;
+    const float *__restrict__ vel = f.velocity;
+    const float *__restrict__ rho = f.density;
+    const V3 xi = load3(pos, i), vi = load3(vel, i);
+    const float rhoi = rho[i];
+    const float pi = pressure(p, rhoi);
+    Acc<KAHAN> fx, fy, fz;
+    for_each_candidate(p.grid, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+        if (j == i) return;
+        const V3 xj = load3(pos, j);
+        const V3 d = xj - xi;
+        const float r = length(d);
+        if (r > kc.h) return;
+        const V3 vj = load3(vel, j);
+        const V3 rv = vj - vi;
+        const float rhoj = rho[j];
+        const float pj = pressure(p, rhoj);
+        const float din = kc.h - r;
+        const float didj = rhoi * rhoj;
+        const float vterm = (p.viscosity * din) * kc.lap;
+        // (p_i + p_j) * 0.5 / r * din * din * spiky, in double
+        // (the 0.5 literal promotes it, kernels/fluids.cl:48)
+        const float pterm = static_cast<float>(static_cast<double>(pi + pj) * 0.5 /
+                                               static_cast<double>(r) * static_cast<double>(din) *
+                                               static_cast<double>(din) * static_cast<double>(kc.spiky));
+        fx.add((d.x * pterm + rv.x * vterm) / didj);
+        fy.add((d.y * pterm + rv.y * vterm) / didj);
+        fz.add((d.z * pterm + rv.z * vterm) / didj);
+    });
+    V3 fi = V3{fx.value(), fy.value(), fz.value()} * p.mass;
+    if (p.mass_consistent) fi = fi * p.mass;   // the reference sums m_j only
+    fi.x = cube_wall(xi.x, fi.x);
+    fi.y = cube_wall(xi.y, fi.y);
+    fi.z = cube_wall(xi.z, fi.z);
+    store3(f.force, i, fi);
+}
+
+__device__ __forceinline__ bool is_static_body(const StepParams &p, const FieldPtrs &f, unsigned i) {
+    if (!f.body_number) return false;
+    const unsigned b = f.body_number[i];
+    if (b == 0u || !f.body_dynamic || b > p.body_count) return false;
+    return f.body_dynamic[b - 1u] == 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_step_forward(StepParams p, FieldPtrs f) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n || is_static_body(p, f, i)) return;
+    const V3 g = V3{p.gravity[0], p.gravity[1], p.gravity[2]};
+    const V3 a = load3(f.force, i) / p.mass + g;
+    if (f.acceleration) store3(f.acceleration, i, a);
+    const V3 v = load3(f.velocity, i);
+    const V3 vn = v + p.dt * a;
+    const V3 x = load3(f.position, i);
+    store3(f.position, i, x + p.dt * vn);
+    store3(f.velocity, i, vn);
+}
+
+__global__ void __launch_bounds__(kBlock) k_plane_constraints(StepParams p, FieldPtrs f) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const unsigned mask = f.plane_constraints_particles[i];
+    if (mask == 0u) return;
+    V3 x = load3(f.position, i), v = load3(f.velocity, i);
+    for (unsigned c = 0; c < p.nconstraints; ++c) {
+        if (!((mask >> c) & 1u)) continue;
+        const V3 pt = V3{p.constraints[c][0], p.constraints[c][1], p.constraints[c][2]};
+        const V3 nr = V3{p.constraints[c][3], p.constraints[c][4], p.constraints[c][5]};
+        x = x - dot(nr, x - pt) * nr;
+        v = v - dot(nr, v) * nr;
+    }
+    store3(f.position, i, x);
+    store3(f.velocity, i, v);
+}
+
+__global__ void __launch_bounds__(kBlock) k_init_original_position(StepParams p, FieldPtrs f) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    store3(f.originalpos, i, load3(f.position, i));
+}
+
+// Deterministic two-level FP64 reduction (fixed grid, fixed tree order).
+__global__ void __launch_bounds__(kBlock) k_sums_partial(StepParams p, FieldPtrs f, double *partials) {
+    __shared__ double sh[7][kBlock];
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const double m = static_cast<double>(p.mass);
+    for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < p.n; i += kReduceBlocks * kBlock) {
+        const double vx = f.velocity ? f.velocity[3 * i] : 0.0;
+        const double vy = f.velocity ? f.velocity[3 * i + 1] : 0.0;
+        const double vz = f.velocity ? f.velocity[3 * i + 2] : 0.0;
+        acc[0] += m * vx;
+        acc[1] += m * vy;
+        acc[2] += m * vz;
+        acc[3] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        acc[4] += m * static_cast<double>(f.position[3 * i]);
+        acc[5] += m * static_cast<double>(f.position[3 * i + 1]);
+        acc[6] += m * static_cast<double>(f.position[3 * i + 2]);
+    }
+    for (int k = 0; k < 7; ++k) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (unsigned w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < 7; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 7; ++k) partials[k * kReduceBlocks + blockIdx.x] = sh[k][0];
+}
+
+__global__ void k_sums_final(const double *partials, double *out7, unsigned n, float mass) {
+    const unsigned k = threadIdx.x;
+    if (k >= 7) return;
+    double s = 0.0;
+    for (unsigned b = 0; b < kReduceBlocks; ++b) s += partials[k * kReduceBlocks + b];
+    if (k >= 4) {
+        const double total = static_cast<double>(mass) * static_cast<double>(n);
+        s = total > 0.0 ? s / total : 0.0;
+    }
+    out7[k] = s;
+}
+
+inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    if (p.compensated)
+        hipLaunchKernelGGL((k_density<false, true>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    else
+        hipLaunchKernelGGL((k_density<false, false>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_compute_original_density(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    if (p.compensated)
+        hipLaunchKernelGGL((k_density<true, true>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    else
+        hipLaunchKernelGGL((k_density<true, false>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_forces_fluids(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    if (p.compensated)
+        hipLaunchKernelGGL((k_forces_fluids<true>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    else
+        hipLaunchKernelGGL((k_forces_fluids<false>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_forward(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_step_forward, grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_plane_constraints(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plane_constraints, grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_original_position(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init_original_position, grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_global_sums(const StepParams &p, const FieldPtrs &f, double *partials, double *out7,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_sums_partial, dim3(kReduceBlocks), dim3(kBlock), 0, s, p, f, partials);
+    hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(64), 0, s, partials, out7, p.n, p.mass);
+    return hipGetLastError();
+}
+
+}  // namespace sphk

@@ -1,0 +1,348 @@
+// Uniform-grid neighbour search: cell keys, per-cell counts, exclusive scan,
+// stable placement (ascending particle id inside each cell) and the
+// cell-sorted reorder of particle state.
+//
+// Reference: kernels/common.cl:372-529 (find_particle_bins, zero_gridcount,
+// count_particles_in_bins, prefix_sum, copy_celloffset_to_backup,
+// insert_particles_in_bin_array) and the host sequence of
+// src/opencl/particle_system_host.c:373-454.  The reference leaves the order
+// inside a cell to atomic arrival; here it is ascending particle index, which
+// is what a sequential run of the reference produces, so gridcell, gridcount,
+// celloffset and cellparticles are deterministic and bit-exact against the
+// sequential oracle.
+#include "kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace sphk {
+namespace {
+
+constexpr unsigned kBlock = 256;
+
+__device__ __forceinline__ unsigned grid_index(float p, float lo, float hi, unsigned res) {
+    // (p - lo) / (hi - lo) * res, truncated toward zero, clamped to [0, res-1]
+    // (kernels/common.cl:384-388).  Values below 0 and NaN land in cell 0.
+    float v = (p - lo) / (hi - lo) * static_cast<float>(res);
+    unsigned u;
+    if (!(v > 0.0f)) u = 0u;
+    else if (v >= 4294967296.0f) u = 0xffffffffu;
+    else u = static_cast<unsigned>(v);
+    return u < res - 1u ? u : res - 1u;
+}
+
+__device__ __forceinline__ unsigned cell_key(const float *pos, unsigned s, const GridGeom &g) {
+    unsigned gx = grid_index(pos[3 * s], g.lo[0], g.hi[0], g.rx);
+    unsigned gy = grid_index(pos[3 * s + 1], g.lo[1], g.hi[1], g.ry);
+    unsigned gz = grid_index(pos[3 * s + 2], g.lo[2], g.hi[2], g.rz);
+    return gz * g.rx * g.ry + gy * g.rx + gx;   // (kernels/common.cl:390)
+}
+
+__global__ void k_fill_u32(unsigned *p, size_t count, unsigned value) {
+    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (; i < count; i += stride) p[i] = value;
+}
+
+__global__ void k_iota_u32(unsigned *p, size_t count) {
+    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (; i < count; i += stride) p[i] = static_cast<unsigned>(i);
+}
+
+__global__ void k_cell_keys(const float *__restrict__ pos, unsigned n, GridGeom g,
+                            unsigned *__restrict__ key) {
+    unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    key[s] = cell_key(pos, s, g);
+}
+
+// Key and arrival rank of every active slot.  Slots are cell-sorted from the
+// previous step, so runs of equal keys inside a wavefront are common; each
+// run issues one atomic and hands out consecutive ranks.
+__global__ void k_key_count(const float *__restrict__ pos, unsigned n, GridGeom g,
+                            unsigned *__restrict__ key, unsigned *__restrict__ local,
+                            unsigned *__restrict__ count) {
+    unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = s < n;
+    unsigned k = active ? cell_key(pos, s, g) : 0xffffffffu;
+    const int lane = threadIdx.x & 63;
+    unsigned prev = __shfl_up(k, 1, 64);
+    const bool head = active && (lane == 0 || prev != k);
+    unsigned long long heads = __ballot(head);
+    unsigned long long act = __ballot(active);
+    if (!active) return;
+    // Run start = highest head lane <= lane; run end = next head (or last active lane).
+    unsigned long long upto = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+    int start = 63 - __clzll(upto);
+    unsigned long long after = heads & ~((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+    int next = after ? __ffsll(static_cast<long long>(after)) - 1 : 64;
+    int last_active = 63 - __clzll(act);
+    int end = next < last_active + 1 ? next : last_active + 1;
+    unsigned base = 0;
+    if (lane == start) base = atomicAdd(&count[k], static_cast<unsigned>(end - start));
+    base = __shfl(base, start, 64);
+    key[s] = k;
+    local[s] = base + static_cast<unsigned>(lane - start);
+}
+
+// ---- exclusive scan over cells -------------------------------------------
+constexpr unsigned kScanItems = kScanTile / kBlock;   // 16 cells per thread
+
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned *total) {
+    __shared__ unsigned wave_sums[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+    }
+    if (lane == 63) wave_sums[wave] = inc;
+    __syncthreads();
+    unsigned wave_off = 0, all = 0;
+    for (int w = 0; w < static_cast<int>(kBlock / 64); ++w) {
+        unsigned ws = wave_sums[w];
+        if (w < wave) wave_off += ws;
+        all += ws;
+    }
+    __syncthreads();
+    *total = all;
+    return wave_off + inc - v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const unsigned *__restrict__ in, unsigned count,
+                                                        unsigned *__restrict__ tile_sums) {
+    const unsigned base = blockIdx.x * kScanTile;
+    unsigned sum = 0;
+    for (unsigned t = threadIdx.x; t < kScanTile; t += kBlock) {
+        unsigned i = base + t;
+        if (i < count) sum += in[i];
+    }
+    unsigned total;
+    block_exclusive_scan(sum, &total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_tile_sums(unsigned *tile_sums, unsigned ntiles) {
+    unsigned carry = 0;
+    for (unsigned base = 0; base < ntiles; base += kBlock) {
+        unsigned i = base + threadIdx.x;
+        unsigned v = i < ntiles ? tile_sums[i] : 0u;
+        unsigned total;
+        unsigned ex = block_exclusive_scan(v, &total);
+        if (i < ntiles) tile_sums[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_final(const unsigned *__restrict__ in, unsigned count,
+                                                       const unsigned *__restrict__ tile_off,
+                                                       unsigned *__restrict__ out) {
+    const unsigned base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    unsigned v[kScanItems];
+    unsigned sum = 0;
+#pragma unroll
+    for (unsigned k = 0; k < kScanItems; ++k) {
+        unsigned i = base + k;
+        v[k] = i < count ? in[i] : 0u;
+        sum += v[k];
+    }
+    unsigned total;
+    unsigned run = tile_off[blockIdx.x] + block_exclusive_scan(sum, &total);
+#pragma unroll
+    for (unsigned k = 0; k < kScanItems; ++k) {
+        unsigned i = base + k;
+        if (i < count) out[i] = run;
+        run += v[k];
+    }
+}
+
+__global__ void k_scatter_unsorted(unsigned n, const unsigned *__restrict__ key,
+                                   const unsigned *__restrict__ local,
+                                   const unsigned *__restrict__ celloffset,
+                                   const unsigned *__restrict__ slot_id, unsigned *__restrict__ tmp_id,
+                                   unsigned *__restrict__ tmp_src, unsigned *__restrict__ tmp_key) {
+    unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    unsigned k = key[s];
+    unsigned t = celloffset[k] + local[s];
+    tmp_id[t] = slot_id[s];
+    tmp_src[t] = s;
+    tmp_key[t] = k;
+}
+
+__global__ void k_rank_place(unsigned n, const unsigned *__restrict__ tmp_id,
+                             const unsigned *__restrict__ tmp_src, const unsigned *__restrict__ tmp_key,
+                             const unsigned *__restrict__ celloffset,
+                             const unsigned *__restrict__ gridcount, unsigned *__restrict__ perm,
+                             unsigned *__restrict__ slot_id_new, unsigned *__restrict__ gridcell,
+                             unsigned *__restrict__ cellparticles, const float *__restrict__ v0s,
+                             float *__restrict__ v0d, const float *__restrict__ v1s,
+                             float *__restrict__ v1d) {
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const unsigned c = tmp_key[t];
+    const unsigned start = celloffset[c], cnt = gridcount[c];
+    const unsigned me = tmp_id[t];
+    unsigned rank = 0;
+    for (unsigned u = start; u < start + cnt; ++u) rank += tmp_id[u] < me ? 1u : 0u;
+    const unsigned f = start + rank;
+    const unsigned src = tmp_src[t];
+    perm[f] = src;
+    slot_id_new[f] = me;
+    gridcell[f] = c;
+    cellparticles[f] = me;
+    if (v0d) {
+        v0d[3 * f] = v0s[3 * src];
+        v0d[3 * f + 1] = v0s[3 * src + 1];
+        v0d[3 * f + 2] = v0s[3 * src + 2];
+    }
+    if (v1d) {
+        v1d[3 * f] = v1s[3 * src];
+        v1d[3 * f + 1] = v1s[3 * src + 1];
+        v1d[3 * f + 2] = v1s[3 * src + 2];
+    }
+}
+
+template <unsigned W>
+__global__ void k_gather_words(const unsigned *__restrict__ src, unsigned *__restrict__ dst,
+                               const unsigned *__restrict__ map, unsigned n, unsigned count) {
+    unsigned f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= count) return;
+    unsigned s = f < n ? map[f] : f;
+#pragma unroll
+    for (unsigned w = 0; w < W; ++w) dst[static_cast<size_t>(f) * W + w] = src[static_cast<size_t>(s) * W + w];
+}
+
+template <unsigned W>
+__global__ void k_scatter_words(const unsigned *__restrict__ src, unsigned *__restrict__ dst,
+                                const unsigned *__restrict__ map, unsigned n, unsigned count) {
+    unsigned f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= count) return;
+    unsigned d = f < n ? map[f] : f;
+#pragma unroll
+    for (unsigned w = 0; w < W; ++w) dst[static_cast<size_t>(d) * W + w] = src[static_cast<size_t>(f) * W + w];
+}
+
+__global__ void k_gather_words_any(const unsigned *__restrict__ src, unsigned *__restrict__ dst,
+                                   const unsigned *__restrict__ map, unsigned n, unsigned count,
+                                   unsigned words) {
+    unsigned f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= count) return;
+    unsigned s = f < n ? map[f] : f;
+    for (unsigned w = 0; w < words; ++w)
+        dst[static_cast<size_t>(f) * words + w] = src[static_cast<size_t>(s) * words + w];
+}
+
+__global__ void k_scatter_words_any(const unsigned *__restrict__ src, unsigned *__restrict__ dst,
+                                    const unsigned *__restrict__ map, unsigned n, unsigned count,
+                                    unsigned words) {
+    unsigned f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= count) return;
+    unsigned d = f < n ? map[f] : f;
+    for (unsigned w = 0; w < words; ++w)
+        dst[static_cast<size_t>(d) * words + w] = src[static_cast<size_t>(f) * words + w];
+}
+
+inline unsigned blocks_for(size_t count) {
+    return static_cast<unsigned>((count + kBlock - 1) / kBlock);
+}
+
+}  // namespace
+
+hipError_t launch_fill_u32(unsigned *p, size_t count, unsigned value, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    unsigned blocks = blocks_for(count);
+    if (blocks > 65536u) blocks = 65536u;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(kBlock), 0, s, p, count, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_iota_u32(unsigned *p, size_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    unsigned blocks = blocks_for(count);
+    if (blocks > 65536u) blocks = 65536u;
+    hipLaunchKernelGGL(k_iota_u32, dim3(blocks), dim3(kBlock), 0, s, p, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_keys(const float *position, unsigned n, const GridGeom &g, unsigned *key_out,
+                            hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cell_keys, dim3(blocks_for(n)), dim3(kBlock), 0, s, position, n, g, key_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_key_count(const float *position, unsigned n, const GridGeom &g, BinScratch &b,
+                            unsigned *gridcount, hipStream_t s) {
+    hipError_t e = launch_fill_u32(gridcount, g.ncells, 0u, s);
+    if (e != hipSuccess || n == 0) return e;
+    hipLaunchKernelGGL(k_key_count, dim3(blocks_for(n)), dim3(kBlock), 0, s, position, n, g, b.key,
+                       b.local, gridcount);
+    return hipGetLastError();
+}
+
+hipError_t launch_exclusive_scan(const unsigned *in, unsigned count, unsigned *out, unsigned *tile_sums,
+                                 hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    unsigned ntiles = (count + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kBlock), 0, s, in, count, tile_sums);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kBlock), 0, s, tile_sums, ntiles);
+    hipLaunchKernelGGL(k_scan_final, dim3(ntiles), dim3(kBlock), 0, s, in, count, tile_sums, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_unsorted(unsigned n, const BinScratch &b, const unsigned *celloffset,
+                                   const unsigned *slot_id, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_unsorted, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, b.key, b.local,
+                       celloffset, slot_id, b.tmp_id, b.tmp_src, b.tmp_key);
+    return hipGetLastError();
+}
+
+hipError_t launch_rank_place(unsigned n, const BinScratch &b, const unsigned *celloffset,
+                             const unsigned *gridcount, unsigned *perm, unsigned *slot_id_new,
+                             unsigned *gridcell, unsigned *cellparticles, const float *vec_src0,
+                             float *vec_dst0, const float *vec_src1, float *vec_dst1, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rank_place, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, b.tmp_id, b.tmp_src,
+                       b.tmp_key, celloffset, gridcount, perm, slot_id_new, gridcell, cellparticles,
+                       vec_src0, vec_dst0, vec_src1, vec_dst1);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_words(const void *src, void *dst, const unsigned *map, unsigned n,
+                               unsigned count, unsigned words, hipStream_t s) {
+    if (count == 0 || words == 0) return hipSuccess;
+    const unsigned *a = static_cast<const unsigned *>(src);
+    unsigned *b = static_cast<unsigned *>(dst);
+    dim3 grid(blocks_for(count)), block(kBlock);
+    switch (words) {
+        case 1: hipLaunchKernelGGL(k_gather_words<1>, grid, block, 0, s, a, b, map, n, count); break;
+        case 3: hipLaunchKernelGGL(k_gather_words<3>, grid, block, 0, s, a, b, map, n, count); break;
+        case 6: hipLaunchKernelGGL(k_gather_words<6>, grid, block, 0, s, a, b, map, n, count); break;
+        case 9: hipLaunchKernelGGL(k_gather_words<9>, grid, block, 0, s, a, b, map, n, count); break;
+        default:
+            hipLaunchKernelGGL(k_gather_words_any, grid, block, 0, s, a, b, map, n, count, words);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_words(const void *src, void *dst, const unsigned *map, unsigned n,
+                                unsigned count, unsigned words, hipStream_t s) {
+    if (count == 0 || words == 0) return hipSuccess;
+    const unsigned *a = static_cast<const unsigned *>(src);
+    unsigned *b = static_cast<unsigned *>(dst);
+    dim3 grid(blocks_for(count)), block(kBlock);
+    switch (words) {
+        case 1: hipLaunchKernelGGL(k_scatter_words<1>, grid, block, 0, s, a, b, map, n, count); break;
+        case 3: hipLaunchKernelGGL(k_scatter_words<3>, grid, block, 0, s, a, b, map, n, count); break;
+        case 6: hipLaunchKernelGGL(k_scatter_words<6>, grid, block, 0, s, a, b, map, n, count); break;
+        case 9: hipLaunchKernelGGL(k_scatter_words<9>, grid, block, 0, s, a, b, map, n, count); break;
+        default:
+            hipLaunchKernelGGL(k_scatter_words_any, grid, block, 0, s, a, b, map, n, count, words);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace sphk

@@ -1,0 +1,437 @@
+// Solid passes: corotational linear elasticity and St. Venant-Kirchhoff.
+//
+// Reference kernels: compute_rotations_and_strains, compute_stresses,
+// compute_forces_solids (kernels/solid_corotation.cl:28-403) and
+// compute_deformations, compute_stresses, compute_forces_solids
+// (kernels/solid_svk.cl:1-367), with the 3x3 helpers of
+// kernels/common.cl:95-350.
+//
+// The polar decomposition (corotational) and the inverse of A_qq (SVK) run
+// in FP64 when StepParams::polar_fp64 is set, in FP32 otherwise; both use the
+// reference's fixed 8-rotation Jacobi.  The SVK Cauchy stress keeps all nine
+// entries (the reference stores the 3x3 product into a 6-float array and
+// reads sigma_zz past its end, kernels/solid_svk.cl:67,75).
+#include "neighbours.h"
+
+#pragma clang fp contract(off)
+
+namespace sphk {
+namespace {
+
+constexpr unsigned kBlock = 256;
+
+template <bool KAHAN>
+struct Acc {
+    float s = 0.0f;
+    __device__ __forceinline__ void add(float v) { s = s + v; }
+    __device__ __forceinline__ float value() const { return s; }
+};
+template <>
+struct Acc<true> {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float v) {
+        float y = v - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ float value() const { return s; }
+};
+
+template <bool KAHAN>
+struct Acc3 {
+    Acc<KAHAN> x, y, z;
+    __device__ __forceinline__ void add(V3 v) { x.add(v.x); y.add(v.y); z.add(v.z); }
+    __device__ __forceinline__ V3 value() const { return V3{x.value(), y.value(), z.value()}; }
+};
+
+// apq += (w p) (x) q   (kernels/solids.cl:66-72)
+template <bool KAHAN>
+__device__ __forceinline__ void add_apq(Acc<KAHAN> *apq, V3 pv, V3 q, float w) {
+    const V3 wp = w * pv;
+    apq[0].add(wp.x * q.x); apq[1].add(wp.x * q.y); apq[2].add(wp.x * q.z);
+    apq[3].add(wp.y * q.x); apq[4].add(wp.y * q.y); apq[5].add(wp.y * q.z);
+    apq[6].add(wp.z * q.x); apq[7].add(wp.z * q.y); apq[8].add(wp.z * q.z);
+}
+// aqq += (w q) (x) q, upper triangle   (kernels/solids.cl:74-80)
+template <bool KAHAN>
+__device__ __forceinline__ void add_aqq(Acc<KAHAN> *aqq, V3 q, float w) {
+    const V3 wq = w * q;
+    aqq[S00].add(wq.x * q.x); aqq[S01].add(wq.x * q.y); aqq[S02].add(wq.x * q.z);
+    aqq[S11].add(wq.y * q.y); aqq[S12].add(wq.y * q.z);
+    aqq[S22].add(wq.z * q.z);
+}
+// out += a (x) b   (kernels/common.cl:252-256)
+template <bool KAHAN>
+__device__ __forceinline__ void add_outer(Acc<KAHAN> *out, V3 a, V3 b) {
+    out[0].add(a.x * b.x); out[1].add(a.x * b.y); out[2].add(a.x * b.z);
+    out[3].add(a.y * b.x); out[4].add(a.y * b.y); out[5].add(a.y * b.z);
+    out[6].add(a.z * b.x); out[7].add(a.z * b.y); out[8].add(a.z * b.z);
+}
+
+// R = A (A^T A)^(-1/2)   (getR, kernels/common.cl:306-331); eigenvalue
+// inverse square roots as 1/sqrt.
+template <typename T>
+__device__ __forceinline__ void polar_rotation(const float *apq, float *rot) {
+    T a[9];
+    for (int k = 0; k < 9; ++k) a[k] = static_cast<T>(apq[k]);
+    T s[6] = {a[0] * a[0] + a[3] * a[3] + a[6] * a[6], a[1] * a[1] + a[4] * a[4] + a[7] * a[7],
+              a[2] * a[2] + a[5] * a[5] + a[8] * a[8], a[0] * a[1] + a[3] * a[4] + a[6] * a[7],
+              a[0] * a[2] + a[3] * a[5] + a[6] * a[8], a[1] * a[2] + a[4] * a[5] + a[7] * a[8]};
+    T ev[3], pv[9], tmp[6], r[9];
+    jacobi8(s, ev, pv);
+    for (int k = 0; k < 3; ++k) ev[k] = T(1) / t_sqrt(ev[k]);
+    p_d_pt(ev, pv, tmp);
+    mul_full_sym(a, tmp, r);
+    for (int k = 0; k < 9; ++k) rot[k] = static_cast<float>(r[k]);
+}
+
+// inverse of symmetric m by Jacobi (jacobiInvert, kernels/common.cl:296-305);
+// in FP32 the reciprocals are taken in double as the 1.0 literal does there.
+__device__ __forceinline__ void recip_ev(float *ev) {
+    for (int k = 0; k < 3; ++k) ev[k] = static_cast<float>(1.0 / static_cast<double>(ev[k]));
+}
+__device__ __forceinline__ void recip_ev(double *ev) {
+    for (int k = 0; k < 3; ++k) ev[k] = 1.0 / ev[k];
+}
+template <typename T>
+__device__ __forceinline__ void deformation_from(const float *apq, const float *aqq, float *def) {
+    T m[6], ev[3], pv[9], inv[6], a[9], out[9];
+    for (int k = 0; k < 6; ++k) m[k] = static_cast<T>(aqq[k]);
+    jacobi8(m, ev, pv);
+    recip_ev(ev);
+    p_d_pt(ev, pv, inv);
+    for (int k = 0; k < 9; ++k) a[k] = static_cast<T>(apq[k]);
+    mul_full_sym(a, inv, out);
+    for (int k = 0; k < 9; ++k) def[k] = static_cast<float>(out[k]);
+}
+
+__device__ __forceinline__ bool in_body(const FieldPtrs &f, unsigned i) {
+    return f.body_number != nullptr && f.body_number[i] > 0u;
+}
+
+template <bool KAHAN, bool FP64>
+__global__ void __launch_bounds__(kBlock) k_rotations_strains(StepParams p, FieldPtrs f) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n || in_body(f, i)) return;
+    const KernelConsts kc = kconsts(p);
+    const float *__restrict__ pos = f.position;
+    const float *__restrict__ opos = f.originalpos;
+    const V3 xi = load3(pos, i), x0i = load3(opos, i);
+    const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
+    const unsigned ncombo = 1u << p.nplanes;
+    const unsigned key = f.gridcell[i];
+
+    Acc<KAHAN> apq[9];
+    for_each_candidate(p.grid, key, f.celloffset, f.gridcount, [&](unsigned j) {
+        const V3 xj = load3(pos, j), x0j = load3(opos, j);
+        if (j != i) {
+            const V3 pv = xj - xi;
+            const float r = length(pv);
+            if (!(r > kc.h)) add_apq(apq, pv, x0j - x0i, w_poly6(r, kc));
+        }
+        if (inear) {
+            const unsigned ok = planes_near_j(p, xj, inear);
+            if (ok)
+                for (unsigned b = 1; b < ncombo; ++b) {
+                    if (b & ~ok) continue;
+                    const V3 pv = reflect_combo(p, b, xj) - xi;
+                    const V3 q = reflect_combo(p, b, x0j) - x0i;
+                    const float d = length(pv);
+                    if (d < kc.h) add_apq(apq, pv, q, w_poly6(d, kc));
+                }
+        }
+    });
+    float a[9];
+    for (int k = 0; k < 9; ++k) a[k] = apq[k].value();
+    if (f.apq_temp)
+        for (int k = 0; k < 9; ++k) f.apq_temp[9 * i + k] = a[k];
+    if (f.intermediate)
+        for (int k = 0; k < 9; ++k) f.intermediate[9 * i + k] = a[k];
+    float rot[9];
+    if (FP64) polar_rotation<double>(a, rot);
+    else polar_rotation<float>(a, rot);
+    for (int k = 0; k < 9; ++k) f.rotation[9 * i + k] = rot[k];
+    const float rt[9] = {rot[0], rot[3], rot[6], rot[1], rot[4], rot[7], rot[2], rot[5], rot[8]};
+
+    const float d0i = f.density0[i];
+    Acc<KAHAN> du[9];
+    for_each_candidate(p.grid, key, f.celloffset, f.gridcount, [&](unsigned j) {
+        const V3 xj = load3(pos, j), x0j = load3(opos, j);
+        if (j != i) {
+            const V3 q = x0j - x0i;
+            if (!(length(q) > kc.h)) {
+                const V3 u = mat_mul(rt, xj - xi) - q;
+                add_outer(du, (u * p.mass) / d0i, grad_spiky(-q, kc));
+            }
+        }
+        if (inear) {
+            const unsigned ok = planes_near_j(p, xj, inear);
+            if (ok)
+                for (unsigned b = 1; b < ncombo; ++b) {
+                    if (b & ~ok) continue;
+                    const V3 pv = mat_mul(rt, reflect_combo(p, b, xj) - xi);
+                    const V3 q = reflect_combo(p, b, x0j) - x0i;
+                    const V3 u = pv - q;
+                    if (length(pv) < kc.h) add_outer(du, (u * p.mass) / d0i, grad_spiky(-q, kc));
+                }
+        }
+    });
+    float g[9];
+    for (int k = 0; k < 9; ++k) g[k] = du[k].value();
+    float *st = f.strain + 6 * i;   // symmetric part (kernels/solid_corotation.cl:1-8)
+    st[S00] = g[0];
+    st[S11] = g[4];
+    st[S22] = g[8];
+    st[S01] = 0.5f * (g[1] + g[3]);
+    st[S02] = 0.5f * (g[2] + g[6]);
+    st[S12] = 0.5f * (g[5] + g[7]);
+}
+
+// sigma = C : eps, isotropic   (kernels/solid_corotation.cl:9-26,197-208)
+__global__ void __launch_bounds__(kBlock) k_stresses_corot(StepParams p, FieldPtrs f) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const float E = p.youngs, nu = p.poisson;
+    const float lam = E * nu / (1.0f + nu) / (1.0f - 2.0f * nu);
+    const float mu = E / 2.0f / (1.0f + nu);
+    const float d = lam + 2.0f * mu, sh = 2.0f * mu;
+    const float c[6][6] = {{d, lam, lam, 0, 0, 0}, {lam, d, lam, 0, 0, 0}, {lam, lam, d, 0, 0, 0},
+                           {0, 0, 0, sh, 0, 0},    {0, 0, 0, 0, sh, 0},    {0, 0, 0, 0, 0, sh}};
+    const float *e = f.strain + 6 * i;
+    float *s = f.stress + 6 * i;
+    float ev[6];
+    for (int k = 0; k < 6; ++k) ev[k] = e[k];
+    for (int r = 0; r < 6; ++r)
+        s[r] = c[r][0] * ev[0] + c[r][1] * ev[1] + c[r][2] * ev[2] + c[r][3] * ev[3] + c[r][4] * ev[4] +
+               c[r][5] * ev[5];
+}
+
+template <bool KAHAN, bool FP64>
+__global__ void __launch_bounds__(kBlock) k_deformations_svk(StepParams p, FieldPtrs f) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const KernelConsts kc = kconsts(p);
+    const float *__restrict__ pos = f.position;
+    const float *__restrict__ opos = f.originalpos;
+    const V3 xi = load3(pos, i), x0i = load3(opos, i);
+    const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
+    const unsigned ncombo = 1u << p.nplanes;
+    Acc<KAHAN> apq[9], aqq[6];
+    for_each_candidate(p.grid, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+        const V3 xj = load3(pos, j), x0j = load3(opos, j);
+        if (j != i) {
+            const V3 pv = xj - xi;
+            const float r = length(pv);
+            if (!(r > kc.h)) {
+                const V3 q = x0j - x0i;
+                const float w = w_poly6(r, kc);
+                add_apq(apq, pv, q, w);
+                add_aqq(aqq, q, w);
+            }
+        }
+        if (inear) {
+            const unsigned ok = planes_near_j(p, xj, inear);
+            if (ok)
+                for (unsigned b = 1; b < ncombo; ++b) {
+                    if (b & ~ok) continue;
+                    const V3 pv = reflect_combo(p, b, xj) - xi;
+                    const V3 q = reflect_combo(p, b, x0j) - x0i;
+                    const float d = length(pv);
+                    if (d < kc.h) {
+                        const float w = w_poly6(d, kc);
+                        add_apq(apq, pv, q, w);
+                        add_aqq(aqq, q, w);
+                    }
+                }
+        }
+    });
+    float a[9], m[6], def[9];
+    for (int k = 0; k < 9; ++k) a[k] = apq[k].value();
+    for (int k = 0; k < 6; ++k) m[k] = aqq[k].value();
+    if (FP64) deformation_from<double>(a, m, def);
+    else deformation_from<float>(a, m, def);
+    for (int k = 0; k < 9; ++k) f.deformation[9 * i + k] = def[k];
+}
+
+// Green strain -> SVK 2nd Piola-Kirchhoff -> Cauchy, with the double
+// literals of kernels/solid_svk.cl:1-43 kept as double arithmetic.
+__global__ void __launch_bounds__(kBlock) k_stresses_svk(StepParams p, FieldPtrs f) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    float F[9], Ft[9], ftf[9];
+    for (int k = 0; k < 9; ++k) F[k] = f.deformation[9 * i + k];
+    Ft[0] = F[0]; Ft[4] = F[4]; Ft[8] = F[8];
+    Ft[1] = F[3]; Ft[3] = F[1]; Ft[2] = F[6]; Ft[6] = F[2]; Ft[5] = F[7]; Ft[7] = F[5];
+    mul_full(Ft, F, ftf);
+    float e[6];
+    e[S00] = static_cast<float>(0.5 * ftf[0] - 0.5);
+    e[S11] = static_cast<float>(0.5 * ftf[4] - 0.5);
+    e[S22] = static_cast<float>(0.5 * ftf[8] - 0.5);
+    e[S01] = static_cast<float>(0.5 * ftf[1]);
+    e[S02] = static_cast<float>(0.5 * ftf[2]);
+    e[S12] = static_cast<float>(0.5 * ftf[5]);
+    const float E = p.youngs, nu = p.poisson;
+    const float lam = static_cast<float>(static_cast<double>(E * nu) / (1.0 + nu) / (1.0 - 2.0 * nu));
+    const float mu = static_cast<float>(E / 2.0 / (1.0 + nu));
+    const float first = lam * (e[S00] + e[S11] + e[S22]);
+    float svk[6];
+    svk[S00] = static_cast<float>(first + 2.0 * mu * e[S00]);
+    svk[S11] = static_cast<float>(first + 2.0 * mu * e[S11]);
+    svk[S22] = static_cast<float>(first + 2.0 * mu * e[S22]);
+    svk[S01] = static_cast<float>(2.0 * mu * e[S01]);
+    svk[S02] = static_cast<float>(2.0 * mu * e[S02]);
+    svk[S12] = static_cast<float>(2.0 * mu * e[S12]);
+    float tmp[9], cauchy[9];
+    mul_full_sym(F, svk, tmp);
+    mul_full(tmp, Ft, cauchy);
+    const float sc = static_cast<float>(1.0 / det3(F));
+    float *s = f.stress + 6 * i;
+    s[S00] = sc * cauchy[0];
+    s[S11] = sc * cauchy[4];
+    s[S22] = sc * cauchy[8];
+    s[S01] = sc * cauchy[1];
+    s[S02] = sc * cauchy[2];
+    s[S12] = sc * cauchy[5];
+}
+
+template <bool CORO, bool KAHAN>
+__global__ void __launch_bounds__(kBlock) k_forces_solids(StepParams p, FieldPtrs f) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const KernelConsts kc = kconsts(p);
+    const float *__restrict__ pos = f.position;
+    const float *__restrict__ opos = f.originalpos;
+    const float *__restrict__ vel = f.velocity;
+    const float *__restrict__ stress = f.stress;
+    const float *__restrict__ rot = f.rotation;
+    const V3 xi = load3(pos, i), vi = load3(vel, i), x0i = load3(opos, i);
+    float si[6], ri[9];
+    for (int k = 0; k < 6; ++k) si[k] = stress[6 * i + k];
+    if (CORO)
+        for (int k = 0; k < 9; ++k) ri[k] = rot[9 * i + k];
+    const float d0i = f.density0[i];
+    const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
+    const unsigned ncombo = 1u << p.nplanes;
+    Acc3<KAHAN> fe, fv, fec, fvc;
+    for_each_candidate(p.grid, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+        const V3 x0j = load3(opos, j);
+        const V3 xj = load3(pos, j);
+        const V3 X = x0i - x0j;
+        const float rx = length(X);
+        const float rcur = length(xj - xi);
+        const bool near_pair = !(rx > kc.h) || !(rcur > kc.h);
+        if (!near_pair && !inear) return;
+        const V3 vj = load3(vel, j);
+        const float vivj = p.mass * p.mass / d0i / f.density0[j];
+        float sj[6], rj[9];
+        for (int k = 0; k < 6; ++k) sj[k] = stress[6 * j + k];
+        if (CORO)
+            for (int k = 0; k < 9; ++k) rj[k] = rot[9 * j + k];
+        const float mrho = p.mass / f.density[j];
+        if (j != i && near_pair) {
+            const V3 fji = -vivj * sym_mul(si, grad_spiky(X, kc));
+            const V3 fij = (f.body_number && f.body_number[j] > 0u) ? -fji
+                                                                    : -vivj * sym_mul(sj, grad_spiky(-X, kc));
+            const V3 e = CORO ? (-mat_mul(ri, fji)) + mat_mul(rj, fij) : (-fji) + fij;
+            const V3 v = (mrho * (vj - vi)) * lap_visc(rcur, kc);
+            fe.add(e);
+            fv.add(v);
+            fec.add(e);
+            fvc.add(v);
+        }
+        if (inear) {
+            const unsigned ok = planes_near_j(p, xj, inear);
+            if (ok)
+                for (unsigned b = 1; b < ncombo; ++b) {
+                    if (b & ~ok) continue;
+                    const V3 rp = reflect_combo(p, b, xj);
+                    const V3 rp0 = reflect_combo(p, b, x0j);
+                    const V3 rip0 = reflect_combo(p, b, x0i);
+                    const V3 Xm = x0i - rp0;
+                    const V3 refx = rip0 - x0j;
+                    const V3 fji = -vivj * sym_mul(si, grad_spiky(Xm, kc));
+                    V3 fij = -vivj * sym_mul(sj, grad_spiky(-refx, kc));
+                    if (CORO) fij = mat_mul(rj, fij);
+                    unsigned lowest = 0;
+                    for (unsigned s = p.nplanes; s > 0; --s)
+                        if ((b >> (s - 1)) & 1u) {
+                            fij = reflect_dir(fij, plane_normal(p, s - 1));
+                            lowest = s - 1;
+                        }
+                    const V3 e = CORO ? (-mat_mul(ri, fji)) + fij : (-fji) + fij;
+                    fe.add(e);
+                    const V3 refvel = reflect_dir(vj, plane_normal(p, lowest));
+                    const V3 v = (mrho * (refvel - vi)) * lap_visc(length(rp - xi), kc);
+                    fv.add(v);
+                    if (!(b & 1u)) {
+                        fec.add(e);
+                        fvc.add(v);
+                    }
+                }
+        }
+    });
+    const V3 e = fe.value() * 0.5f;
+    const V3 v = fv.value() * p.viscosity;
+    store3(f.force, i, e + v);
+    if (f.force_elastic) store3(f.force_elastic, i, e);
+    if (f.force_viscous) store3(f.force_viscous, i, v);
+    if (f.cutoff_force) store3(f.cutoff_force, i, fec.value() * 0.5f + fvc.value() * p.viscosity);
+}
+
+inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+hipError_t launch_rotations_and_strains(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    dim3 g = grid_for(p.n), b(kBlock);
+    if (p.compensated) {
+        if (p.polar_fp64) hipLaunchKernelGGL((k_rotations_strains<true, true>), g, b, 0, s, p, f);
+        else hipLaunchKernelGGL((k_rotations_strains<true, false>), g, b, 0, s, p, f);
+    } else {
+        if (p.polar_fp64) hipLaunchKernelGGL((k_rotations_strains<false, true>), g, b, 0, s, p, f);
+        else hipLaunchKernelGGL((k_rotations_strains<false, false>), g, b, 0, s, p, f);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stresses_corotational(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stresses_corot, grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_deformations_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    dim3 g = grid_for(p.n), b(kBlock);
+    if (p.compensated) {
+        if (p.polar_fp64) hipLaunchKernelGGL((k_deformations_svk<true, true>), g, b, 0, s, p, f);
+        else hipLaunchKernelGGL((k_deformations_svk<true, false>), g, b, 0, s, p, f);
+    } else {
+        if (p.polar_fp64) hipLaunchKernelGGL((k_deformations_svk<false, true>), g, b, 0, s, p, f);
+        else hipLaunchKernelGGL((k_deformations_svk<false, false>), g, b, 0, s, p, f);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stresses_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stresses_svk, grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_forces_solids(const StepParams &p, const FieldPtrs &f, bool corotational, hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    dim3 g = grid_for(p.n), b(kBlock);
+    if (corotational) {
+        if (p.compensated) hipLaunchKernelGGL((k_forces_solids<true, true>), g, b, 0, s, p, f);
+        else hipLaunchKernelGGL((k_forces_solids<true, false>), g, b, 0, s, p, f);
+    } else {
+        if (p.compensated) hipLaunchKernelGGL((k_forces_solids<false, true>), g, b, 0, s, p, f);
+        else hipLaunchKernelGGL((k_forces_solids<false, false>), g, b, 0, s, p, f);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace sphk

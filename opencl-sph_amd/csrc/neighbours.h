@@ -1,0 +1,85 @@
+// Neighbour iteration over the cell-sorted slots and the mirror-plane tests.
+//
+// Particle state lives in slot order: slots [celloffset[c], celloffset[c] +
+// gridcount[c]) hold the particles of cell c in ascending particle index.
+// The reference visits the 27 cells around a particle in (z, y, x) offset
+// order, skipping cells outside the grid, and each cell's particles in
+// cellparticles order (kernels/common.cl:49-68, 354-371).  The three x-cells
+// of one (z, y) row are consecutive in slot space, so the same visiting order
+// is 9 contiguous slot ranges.
+#pragma once
+
+#include "device_math.h"
+#include "kernels.h"
+
+namespace sphk {
+
+template <typename Body>
+__device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned key,
+                                                   const unsigned *__restrict__ celloffset,
+                                                   const unsigned *__restrict__ gridcount,
+                                                   Body &&body) {
+    const unsigned zo = key / g.layer;
+    const unsigned rem = key - g.layer * zo;
+    const unsigned yo = rem / g.rx;
+    const unsigned xo = rem - g.rx * yo;
+    const unsigned x0 = xo > 0u ? xo - 1u : 0u;
+    const unsigned x1 = xo + 1u < g.rx ? xo + 1u : xo;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int zn = static_cast<int>(zo) + dz;
+        if (zn < 0 || zn >= static_cast<int>(g.rz)) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yn = static_cast<int>(yo) + dy;
+            if (yn < 0 || yn >= static_cast<int>(g.ry)) continue;
+            const unsigned row = static_cast<unsigned>(zn) * g.layer + static_cast<unsigned>(yn) * g.rx;
+            const unsigned b = celloffset[row + x0];
+            const unsigned e = celloffset[row + x1] + gridcount[row + x1];
+            for (unsigned j = b; j < e; ++j) body(j);
+        }
+    }
+}
+
+__device__ __forceinline__ V3 plane_point(const StepParams &p, unsigned s) {
+    return V3{p.planes[s][0], p.planes[s][1], p.planes[s][2]};
+}
+__device__ __forceinline__ V3 plane_normal(const StepParams &p, unsigned s) {
+    return V3{p.planes[s][3], p.planes[s][4], p.planes[s][5]};
+}
+
+// Planes (with a non-zero normal) that particle i is within h of.  A plane
+// combination can only contribute a mirror image when every plane in it is
+// in this set (kernels/common.cl:658-670: any plane failing the test skips
+// the combination, and a zero normal always fails the 1e-5 test).
+__device__ __forceinline__ unsigned planes_near_i(const StepParams &p, V3 xi) {
+    unsigned m = 0;
+    for (unsigned s = 0; s < p.nplanes; ++s) {
+        if (!((p.active_planes >> s) & 1u)) continue;
+        float d = fabsf(dot(plane_normal(p, s), xi - plane_point(p, s)));
+        if (!(d > p.h)) m |= 1u << s;
+    }
+    return m;
+}
+// Planes of `cand` whose distance to j lies in [1e-5, h].  The 1e-5 bound is
+// a double literal in the reference, so the comparison is done in double.
+__device__ __forceinline__ unsigned planes_near_j(const StepParams &p, V3 xj, unsigned cand) {
+    unsigned m = 0;
+    for (unsigned s = 0; s < p.nplanes; ++s) {
+        if (!((cand >> s) & 1u)) continue;
+        float d = fabsf(dot(plane_normal(p, s), xj - plane_point(p, s)));
+        if (!(d > p.h || static_cast<double>(d) < 1.0e-5)) m |= 1u << s;
+    }
+    return m;
+}
+
+// Position reflected in every plane of combination b, lowest plane first.
+__device__ __forceinline__ V3 reflect_combo(const StepParams &p, unsigned b, V3 x) {
+    for (unsigned s = 0; s < p.nplanes; ++s)
+        if ((b >> s) & 1u) x = reflect_point(x, plane_point(p, s), plane_normal(p, s));
+    return x;
+}
+
+__device__ __forceinline__ KernelConsts kconsts(const StepParams &p) {
+    return KernelConsts{p.h, p.h2, p.poly6, p.spiky, p.lap};
+}
+
+}  // namespace sphk

@@ -1,0 +1,224 @@
+"""ctypes declarations for libsph_hip.so (include/sph_hip.h).
+
+The psdata struct and the host-side functions keep the layout and names of
+the reference's generated bindings (scripts/sph/bindings.py:625-760); the
+device functions are the HIP replacements of the OpenCL ones
+(scripts/sph/bindings.py:961-1062).  Device calls return a status; `check`
+turns a non-zero status into SphError carrying the library's message.
+"""
+import ctypes
+import os
+from ctypes import (POINTER, Structure, Union, c_char, c_char_p, c_double, c_float, c_int, c_size_t,
+                    c_uint, c_ulonglong, c_void_p)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.environ.get("SPH_HIP_LIBRARY", os.path.join(HERE, "libsph_hip.so"))
+
+
+class SphError(RuntimeError):
+    pass
+
+
+if not os.path.exists(LIBRARY):
+    raise ImportError(
+        f"{LIBRARY} is missing: build it with `python opencl-sph_amd/build.py` "
+        "(there is no CPU fallback for the simulation step)")
+
+_lib = ctypes.CDLL(LIBRARY)
+
+# FieldType (src/datatypes.h:4-11)
+T_CHAR, T_UINT, T_INT, T_FLOAT32, T_FLOAT64, TYPE_COUNT = range(6)
+
+
+class _DataPtrUnion(Union):
+    _fields_ = [("c", POINTER(c_char)), ("u", POINTER(c_uint)), ("i", POINTER(ctypes.c_int32)),
+                ("f4", POINTER(c_float)), ("f8", POINTER(c_double)), ("ptr", c_void_p)]
+
+
+class DataPtr(Structure):
+    _anonymous_ = ("_u",)
+    _fields_ = [("type", c_int), ("_u", _DataPtrUnion)]
+
+
+class psdata(Structure):
+    _fields_ = [
+        ("num_fields", c_uint),
+        ("names", c_char_p),
+        ("names_offsets", POINTER(c_uint)),
+        ("dimensions", POINTER(c_uint)),
+        ("num_dimensions", POINTER(c_uint)),
+        ("dimensions_offsets", POINTER(c_uint)),
+        ("entry_sizes", POINTER(c_uint)),
+        ("data", c_void_p),
+        ("data_sizes", POINTER(c_uint)),
+        ("data_offsets", POINTER(c_uint)),
+        ("data_ptrs", POINTER(DataPtr)),
+        ("num_host_fields", c_uint),
+        ("host_names", POINTER(POINTER(c_char))),
+        ("host_data", POINTER(c_void_p)),
+        ("host_data_size", POINTER(c_uint)),
+    ]
+
+
+def _fn(name, argtypes, restype):
+    f = getattr(_lib, name)
+    f.argtypes = argtypes
+    f.restype = restype
+    return f
+
+
+# ---- logging, config, psdata (same names and signatures as libSPH.so) -----
+set_log_level = _fn("set_log_level", [c_uint], None)
+load_config = _fn("load_config", [c_char_p], c_int)
+_get_config_section = _fn("get_config_section", [c_char_p], ctypes.c_char_p)
+unload_config = _fn("unload_config", [], None)
+build_psdata_from_string = _fn("build_psdata_from_string", [POINTER(psdata), c_char_p], None)
+build_psdata = _fn("build_psdata", [POINTER(psdata), c_char_p], None)
+psdata_last_build_status = _fn("psdata_last_build_status", [], c_int)
+get_field_psdata = _fn("get_field_psdata", [psdata, c_char_p], c_int)
+set_field_psdata = _fn("set_field_psdata", [POINTER(psdata), c_char_p, c_void_p, c_uint, c_uint], None)
+psdata_names_size = _fn("psdata_names_size", [psdata], c_uint)
+psdata_dimensions_size = _fn("psdata_dimensions_size", [psdata], c_uint)
+psdata_data_size = _fn("psdata_data_size", [psdata], c_uint)
+create_host_field_psdata = _fn("create_host_field_psdata", [POINTER(psdata), c_char_p, c_void_p, c_uint], c_int)
+get_host_field_psdata = _fn("get_host_field_psdata", [POINTER(psdata), c_char_p], c_int)
+free_psdata = _fn("free_psdata", [POINTER(psdata)], None)
+
+
+def get_config_section(heading):
+    """Section body as text (None when absent)."""
+    if isinstance(heading, str):
+        heading = heading.encode()
+    raw = _get_config_section(heading)
+    return None if raw is None else raw.decode()
+
+
+# ---- device runtime ---------------------------------------------------------
+_last_error = _fn("sph_hip_last_error", [], c_char_p)
+
+
+def check(status):
+    if status != 0:
+        msg = _last_error()
+        raise SphError(msg.decode() if msg else f"libsph_hip call failed with status {status}")
+    return status
+
+
+_init_hip = _fn("init_hip", [], c_int)
+_terminate_hip = _fn("terminate_hip", [], c_int)
+_create_psdata_hip = _fn("create_psdata_hip", [POINTER(psdata), c_char_p, POINTER(c_void_p)], c_int)
+_free_psdata_hip = _fn("free_psdata_hip", [c_void_p], c_int)
+_sync_h2d = _fn("sync_psdata_host_to_device_hip", [psdata, c_void_p, c_int], c_int)
+_sync_d2h = _fn("sync_psdata_device_to_host_hip", [psdata, c_void_p], c_int)
+_sync_fields_h2d = _fn("sync_psdata_fields_host_to_device_hip", [psdata, c_void_p, c_size_t, POINTER(c_char_p)], c_int)
+_sync_fields_d2h = _fn("sync_psdata_fields_device_to_host_hip", [psdata, c_void_p, c_size_t, POINTER(c_char_p)], c_int)
+_call_for_all = _fn("call_for_all_particles_device_hip", [c_void_p, c_char_p], c_int)
+_bins = _fn("compute_particle_bins_device_hip", [c_void_p], c_int)
+_density = _fn("compute_density_device_hip", [c_void_p], c_int)
+_forces = _fn("compute_forces_device_hip", [c_void_p], c_int)
+_step_forward = _fn("step_forward_device_hip", [c_void_p], c_int)
+_step = _fn("step_device_hip", [c_void_p, c_char_p, c_uint, c_int], c_int)
+_synchronize = _fn("synchronize_hip", [c_void_p], c_int)
+_set_precision = _fn("set_precision_hip", [c_void_p, c_int, c_int], c_int)
+_set_profiling = _fn("set_profiling_hip", [c_void_p, c_int], c_int)
+_reset_stats = _fn("reset_kernel_stats_hip", [c_void_p], c_int)
+_get_stats = _fn("get_kernel_stats_hip", [c_void_p, c_int, POINTER(c_char_p), POINTER(c_ulonglong),
+                                          POINTER(c_double)], c_int)
+_global_sums = _fn("global_sums_hip", [c_void_p, POINTER(c_double)], c_int)
+
+
+def _names_array(names):
+    arr = (c_char_p * len(names))()
+    arr[:] = [n.encode() if isinstance(n, str) else n for n in names]
+    return arr
+
+
+def init_hip():
+    check(_init_hip())
+
+
+def terminate_hip():
+    check(_terminate_hip())
+
+
+class PsdataHip:
+    """Device handle for one psdata (the HIP counterpart of psdata_opencl)."""
+
+    def __init__(self, data, file_list):
+        self._data = data
+        handle = c_void_p()
+        if isinstance(file_list, str):
+            file_list = file_list.encode()
+        check(_create_psdata_hip(ctypes.byref(data), file_list, ctypes.byref(handle)))
+        self.handle = handle
+
+    def __bool__(self):
+        return bool(self.handle)
+
+    def free(self):
+        if self.handle:
+            check(_free_psdata_hip(self.handle))
+            self.handle = c_void_p()
+
+    def sync_host_to_device(self, data, full=0):
+        check(_sync_h2d(data, self.handle, int(full)))
+
+    def sync_device_to_host(self, data):
+        check(_sync_d2h(data, self.handle))
+
+    def sync_fields_host_to_device(self, data, names):
+        arr = _names_array(names)
+        check(_sync_fields_h2d(data, self.handle, len(names), arr))
+
+    def sync_fields_device_to_host(self, data, names):
+        arr = _names_array(names)
+        check(_sync_fields_d2h(data, self.handle, len(names), arr))
+
+    def call_for_all_particles(self, kernel):
+        check(_call_for_all(self.handle, kernel.encode() if isinstance(kernel, str) else kernel))
+
+    def compute_particle_bins(self):
+        check(_bins(self.handle))
+
+    def compute_density(self):
+        check(_density(self.handle))
+
+    def compute_forces(self):
+        check(_forces(self.handle))
+
+    def step_forward(self):
+        check(_step_forward(self.handle))
+
+    def step(self, system, num_steps=1, apply_constraints=False):
+        check(_step(self.handle, system.encode(), int(num_steps), int(bool(apply_constraints))))
+
+    def synchronize(self):
+        check(_synchronize(self.handle))
+
+    def set_precision(self, compensated_sums=False, polar_fp64=True):
+        check(_set_precision(self.handle, int(bool(compensated_sums)), int(bool(polar_fp64))))
+
+    def set_profiling(self, enabled=True):
+        check(_set_profiling(self.handle, int(bool(enabled))))
+
+    def reset_kernel_stats(self):
+        check(_reset_stats(self.handle))
+
+    def kernel_stats(self):
+        """{kernel name: (launches, total milliseconds)} from HIP events."""
+        count = _get_stats(self.handle, 0, None, None, None)
+        if count < 0:
+            check(-1)
+        names = (c_char_p * max(count, 1))()
+        launches = (c_ulonglong * max(count, 1))()
+        total = (c_double * max(count, 1))()
+        count = _get_stats(self.handle, count, names, launches, total)
+        if count < 0:
+            check(-1)
+        return {names[k].decode(): (int(launches[k]), float(total[k])) for k in range(count)}
+
+    def global_sums(self):
+        """(momentum[3], kinetic energy, centre of mass[3]) reduced in FP64."""
+        out = (c_double * 7)()
+        check(_global_sums(self.handle, out))
+        return list(out[0:3]), out[3], list(out[4:7])

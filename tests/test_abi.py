@@ -1,0 +1,49 @@
+"""The C ABI library loads and exports every function include/sph_hip.h
+declares (no device calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+HEADER = os.path.join(REPO, "include", "sph_hip.h")
+LIB = os.path.join(REPO, "opencl-sph_amd", "sph", "libsph_hip.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?(?:unsigned\s+)?\w+\s*\**\s*(\w+)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while")))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for required in ("build_psdata_from_string", "load_config", "init_hip", "create_psdata_hip",
+                     "call_for_all_particles_device_hip", "compute_particle_bins_device_hip",
+                     "sync_psdata_fields_device_to_host_hip", "step_device_hip"):
+        assert required in names
+    assert len(names) >= 35
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_package_imports_and_reports_errors_without_device():
+    import sph
+    from sph import bindings as bnd
+    assert bnd.load_config(b"/nonexistent.conf") == 1
+    ps = sph.init_data_cpu("pnum | 1 | unsigned int | 4\nn | 1 | unsigned int\nsmoothingradius | 1 | float | 0.5\n"
+                           "gridbounds | 2, 3 | float | 0, 1, 0, 1, 0, 1\nposition | 3, pnum | float\n")
+    assert bnd.get_field_psdata(ps, b"position") >= 0
+    assert bnd.get_field_psdata(ps, b"missing") == -1
+    sph.free_data_cpu(ps)
+    try:
+        sph.init_data_cpu("")
+    except sph.SphError:
+        pass
+    else:
+        raise AssertionError("empty specification must raise")
