@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kBlock) k_density(StepParams p, FieldPtrs f) {
     const unsigned ncombo = 1u << p.nplanes;
     Acc<KAHAN> rho;
     unsigned nn = 0;
-    for_each_candidate(p.grid, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+    for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j);
         const float r = length(xj - xi);
         if (!(r > kc.h)) rho.add(w_poly6(r, kc));
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(StepParams p, FieldPtr
     const float rhoi = rho[i];
     const float pi = pressure(p, rhoi);
     Acc<KAHAN> fx, fy, fz;
-    for_each_candidate(p.grid, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+    for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         if (j == i) return;
         const V3 xj = load3(pos, j);
         const V3 d = xj - xi;

@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(StepParams p, Fiel
     const unsigned key = f.gridcell[i];
 
     Acc<KAHAN> apq[9];
-    for_each_candidate(p.grid, key, f.celloffset, f.gridcount, [&](unsigned j) {
+    for_each_candidate(p.grid, p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 pv = xj - xi;
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(StepParams p, Fiel
 
     const float d0i = f.density0[i];
     Acc<KAHAN> du[9];
-    for_each_candidate(p.grid, key, f.celloffset, f.gridcount, [&](unsigned j) {
+    for_each_candidate(p.grid, p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 q = x0j - x0i;
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) k_deformations_svk(StepParams p, Field
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned ncombo = 1u << p.nplanes;
     Acc<KAHAN> apq[9], aqq[6];
-    for_each_candidate(p.grid, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+    for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 pv = xj - xi;
@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(StepParams p, FieldPtr
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned ncombo = 1u << p.nplanes;
     Acc3<KAHAN> fe, fv, fec, fvc;
-    for_each_candidate(p.grid, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+    for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 x0j = load3(opos, j);
         const V3 xj = load3(pos, j);
         const V3 X = x0i - x0j;

@@ -6,7 +6,7 @@
 // order, skipping cells outside the grid, and each cell's particles in
 // cellparticles order (kernels/common.cl:49-68, 354-371).  The three x-cells
 // of one (z, y) row are consecutive in slot space, so the same visiting order
-// is 9 contiguous slot ranges.
+// is 9 contiguous slot ranges.  Keys outside the grid visit no cell.
 #pragma once
 
 #include "device_math.h"
@@ -15,10 +15,11 @@
 namespace sphk {
 
 template <typename Body>
-__device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned key,
+__device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned n, unsigned key,
                                                    const unsigned *__restrict__ celloffset,
                                                    const unsigned *__restrict__ gridcount,
                                                    Body &&body) {
+    if (key >= g.ncells) return;
     const unsigned zo = key / g.layer;
     const unsigned rem = key - g.layer * zo;
     const unsigned yo = rem / g.rx;
@@ -32,8 +33,10 @@ __device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned k
             const int yn = static_cast<int>(yo) + dy;
             if (yn < 0 || yn >= static_cast<int>(g.ry)) continue;
             const unsigned row = static_cast<unsigned>(zn) * g.layer + static_cast<unsigned>(yn) * g.rx;
-            const unsigned b = celloffset[row + x0];
-            const unsigned e = celloffset[row + x1] + gridcount[row + x1];
+            // Ranges are clamped to the active slots so that grid arrays
+            // uploaded from the host can never index past the particle data.
+            const unsigned b = min(celloffset[row + x0], n);
+            const unsigned e = min(celloffset[row + x1] + gridcount[row + x1], n);
             for (unsigned j = b; j < e; ++j) body(j);
         }
     }

@@ -776,3 +776,15 @@ void orc_forces_solids(const orc_params *p, const real *pos, const real *vel, co
 }
 
 unsigned orc_real_size(void) { return (unsigned)sizeof(real); }
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+/* Threads the OpenMP passes use (1 without OpenMP). */
+unsigned orc_threads(void) {
+#ifdef _OPENMP
+    return (unsigned)omp_get_max_threads();
+#else
+    return 1u;
+#endif
+}
