@@ -36,7 +36,8 @@ __device__ __forceinline__ V3 operator*(float s, V3 a) { return V3{s * a.x, s * 
 __device__ __forceinline__ V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ V3 operator/(V3 a, float s) { return V3{a.x / s, a.y / s, a.z / s}; }
 __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float length(V3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ float sqlength(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+__device__ __forceinline__ float length(V3 a) { return sqrtf(sqlength(a)); }
 
 // Reflection of point `a` in the plane through `pt` with normal `nrm`:
 // a - 2 n (n . (a - pt))   (kernels/common.cl:672).

@@ -22,6 +22,10 @@ struct StepParams {
     unsigned n;               // active particles (slots [0, n))
     GridGeom grid;
     float h, h2, poly6, spiky, lap;
+    // Smallest float s with RN(sqrt(s)) > h: for s = squared distance,
+    // "length > h" holds exactly when s >= r2_out (sqrt is correctly rounded
+    // and monotonic), so out-of-range candidates need no square root.
+    float r2_out;
     float mass, dt;
     float restdens, stiffness, viscosity;
     float gravity[3];
@@ -91,7 +95,24 @@ hipError_t launch_scatter_words(const void *src, void *dst, const unsigned *map,
                                 unsigned count, unsigned words, hipStream_t s);
 
 // ---- per-particle passes (kernels_fluid.hip, kernels_solid.hip) ------------
+// In-range neighbour lists written by the density pass: for slot i, entries
+// k < count[i] at list[k * n + i] are the slots j with length(x_j - x_i) <= h
+// in the reference's visiting order (self included).  count[i] > kListCap
+// means the list overflowed and readers must scan the candidates instead.
+constexpr unsigned kListCap = 48;
+struct PairCache {
+    unsigned *list;        // [kListCap * pnum]
+    unsigned *count;       // [pnum]
+    float *pressure;       // [pnum] equation of state of each slot's density
+    int valid;             // written by the density pass of the current state
+};
+
 hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+// Density that also fills the pair cache (fluid programs).
+hipError_t launch_compute_density_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
+                                         hipStream_t s);
+hipError_t launch_forces_fluids_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
+                                       hipStream_t s);
 hipError_t launch_compute_original_density(const StepParams &p, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_forces_fluids(const StepParams &p, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_step_forward(const StepParams &p, const FieldPtrs &f, hipStream_t s);

@@ -77,6 +77,82 @@ __global__ void __launch_bounds__(kBlock) k_density(StepParams p, FieldPtrs f) {
     }
 }
 
+__device__ __forceinline__ float pressure(const StepParams &p, float rho);
+
+// compute_density for fluid programs, in two phases per lane:
+//   1. scan the candidates with the square-root-free range test and compact
+//      the in-range slots into this lane's column of an LDS buffer;
+//   2. evaluate the kernel sum over the compacted slots (every lane busy),
+//      writing them out as the pair cache's neighbour list.
+// The summation order is the candidate order, as in the reference.  Lanes
+// near a mirror plane, or with more than kListCap in-range candidates, take
+// the reference loop instead.
+constexpr unsigned kListBlock = 128;
+
+template <bool KAHAN>
+__global__ void __launch_bounds__(kListBlock) k_density_listed(StepParams p, FieldPtrs f, PairCache c) {
+    __shared__ unsigned buf[kListCap * kListBlock];
+    const unsigned i = blockIdx.x * kListBlock + threadIdx.x;
+    if (i >= p.n) return;
+    const KernelConsts kc = kconsts(p);
+    const float *__restrict__ pos = f.position;
+    const V3 xi = load3(pos, i);
+    const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
+    const unsigned key = f.gridcell[i];
+    unsigned *col = buf + threadIdx.x;
+    unsigned cnt = 0;
+    if (!inear) {
+        for_each_candidate(p.grid, p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
+            const float s = sqlength(load3(pos, j) - xi);
+            if (!(s >= p.r2_out)) {
+                if (cnt < kListCap) col[cnt * kListBlock] = j;
+                ++cnt;
+            }
+        });
+    }
+    Acc<KAHAN> rho;
+    unsigned nn = 0;
+    if (!inear && cnt <= kListCap) {
+        for (unsigned k = 0; k < cnt; ++k) {
+            const unsigned j = col[k * kListBlock];
+            const float r = length(load3(pos, j) - xi);
+            rho.add(w_poly6(r, kc));
+            if (r < kc.h) ++nn;
+            c.list[static_cast<size_t>(k) * p.n + i] = j;
+        }
+    } else {
+        const unsigned ncombo = 1u << p.nplanes;
+        cnt = 0;
+        for_each_candidate(p.grid, p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
+            const V3 xj = load3(pos, j);
+            const float r = length(xj - xi);
+            if (!(r > kc.h)) {
+                rho.add(w_poly6(r, kc));
+                if (cnt < kListCap) c.list[static_cast<size_t>(cnt) * p.n + i] = j;
+                ++cnt;
+            }
+            if (r < kc.h) ++nn;
+            if (inear) {
+                const unsigned ok = planes_near_j(p, xj, inear);
+                if (ok)
+                    for (unsigned b = 1; b < ncombo; ++b) {
+                        if (b & ~ok) continue;
+                        const float rr = length(reflect_combo(p, b, xj) - xi);
+                        if (rr < kc.h) {
+                            rho.add(w_poly6(rr, kc));
+                            ++nn;
+                        }
+                    }
+            }
+        });
+    }
+    const float out = rho.value() * p.mass;
+    f.density[i] = out;
+    if (f.num_neighbours) f.num_neighbours[i] = nn;
+    c.count[i] = cnt;
+    c.pressure[i] = pressure(p, out);
+}
+
 // Equation of state: the reference's linear p = k (rho - rho0)
 // (kernels/fluids.cl:28-29), or Tait with exponent 7 when the conf sets
 // eos_tait.
@@ -135,6 +211,58 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(StepParams p, FieldPtr
     });
     V3 fi = V3{fx.value(), fy.value(), fz.value()} * p.mass;
     if (p.mass_consistent) fi = fi * p.mass;   // the reference sums m_j only
+    fi.x = cube_wall(xi.x, fi.x);
+    fi.y = cube_wall(xi.y, fi.y);
+    fi.z = cube_wall(xi.z, fi.z);
+    store3(f.force, i, fi);
+}
+
+// compute_forces_fluids over the pair cache: the in-range list of each slot
+// and the pressures of the density pass (identical values to evaluating the
+// equation of state per pair).  Overflowed lists fall back to the scan.
+template <bool KAHAN>
+__global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(StepParams p, FieldPtrs f, PairCache c) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const KernelConsts kc = kconsts(p);
+    const float *__restrict__ pos = f.position;
+    const float *__restrict__ vel = f.velocity;
+    const float *__restrict__ rho = f.density;
+    const float *__restrict__ prs = c.pressure;
+    const V3 xi = load3(pos, i), vi = load3(vel, i);
+    const float rhoi = rho[i];
+    const float pi = prs[i];
+    Acc<KAHAN> fx, fy, fz;
+    auto pair = [&](unsigned j) {
+        const V3 d = load3(pos, j) - xi;
+        const float r = length(d);
+        if (r > kc.h) return;
+        const V3 rv = load3(vel, j) - vi;
+        const float rhoj = rho[j];
+        const float pj = prs[j];
+        const float din = kc.h - r;
+        const float didj = rhoi * rhoj;
+        const float vterm = (p.viscosity * din) * kc.lap;
+        const float pterm = static_cast<float>(static_cast<double>(pi + pj) * 0.5 /
+                                               static_cast<double>(r) * static_cast<double>(din) *
+                                               static_cast<double>(din) * static_cast<double>(kc.spiky));
+        fx.add((d.x * pterm + rv.x * vterm) / didj);
+        fy.add((d.y * pterm + rv.y * vterm) / didj);
+        fz.add((d.z * pterm + rv.z * vterm) / didj);
+    };
+    const unsigned cnt = c.count[i];
+    if (cnt <= kListCap) {
+        for (unsigned k = 0; k < cnt; ++k) {
+            const unsigned j = c.list[static_cast<size_t>(k) * p.n + i];
+            if (j != i) pair(j);
+        }
+    } else {
+        for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+            if (j != i) pair(j);
+        });
+    }
+    V3 fi = V3{fx.value(), fy.value(), fz.value()} * p.mass;
+    if (p.mass_consistent) fi = fi * p.mass;
     fi.x = cube_wall(xi.x, fi.x);
     fi.y = cube_wall(xi.y, fi.y);
     fi.z = cube_wall(xi.z, fi.z);
@@ -234,6 +362,27 @@ hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipSt
         hipLaunchKernelGGL((k_density<false, true>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
     else
         hipLaunchKernelGGL((k_density<false, false>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_compute_density_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
+                                         hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    const dim3 grid((p.n + kListBlock - 1) / kListBlock), block(kListBlock);
+    if (p.compensated)
+        hipLaunchKernelGGL((k_density_listed<true>), grid, block, 0, s, p, f, c);
+    else
+        hipLaunchKernelGGL((k_density_listed<false>), grid, block, 0, s, p, f, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_forces_fluids_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
+                                       hipStream_t s) {
+    if (p.n == 0) return hipSuccess;
+    if (p.compensated)
+        hipLaunchKernelGGL((k_forces_fluids_listed<true>), grid_for(p.n), dim3(kBlock), 0, s, p, f, c);
+    else
+        hipLaunchKernelGGL((k_forces_fluids_listed<false>), grid_for(p.n), dim3(kBlock), 0, s, p, f, c);
     return hipGetLastError();
 }
 

@@ -103,6 +103,7 @@ struct psdata_hip {
     void *staging = nullptr;
     size_t staging_bytes = 0;
     double *sums = nullptr;                 // partials + 7 outputs
+    sphk::PairCache pairs{};                // neighbour lists + pressures (fluid programs)
     int compensated = 0, polar_fp64 = 1;
     bool profiling = false;
     std::map<std::string, KernelStat> stats;
@@ -181,6 +182,16 @@ int build_params(psdata_hip *h, sphk::StepParams *pp) {
     p.poly6 = static_cast<float>(315.0 / (64.0 * pi * h9));
     p.spiky = static_cast<float>(-45.0 / (pi * h6));
     p.lap = static_cast<float>(45.0 / (pi * h6));
+    // Smallest float s with sqrtf(s) > h (sqrtf is correctly rounded on the
+    // host and on the device, and monotonic).
+    float s2 = hh * hh;
+    if (hh > 0.0f && std::isfinite(hh)) {
+        while (s2 > 0.0f && std::sqrt(s2) > hh) s2 = std::nextafter(s2, 0.0f);
+        while (!(std::sqrt(s2) > hh)) s2 = std::nextafter(s2, INFINITY);
+    } else {
+        s2 = INFINITY;
+    }
+    p.r2_out = s2;
     p.mass = scalar_or(h, "mass", 0.0f);
     p.dt = scalar_or(h, "timestep", 0.0f);
     p.restdens = scalar_or(h, "restdens", 0.0f);
@@ -344,6 +355,7 @@ int reset_permutation(psdata_hip *h) {
 
 // ---- binning ---------------------------------------------------------------
 int compute_bins(psdata_hip *h) {
+    h->pairs.valid = 0;
     if (!h->has_grid) return fail("the psdata has no grid arrays (smoothingradius, gridbounds and pnum need values)");
     DevField *pos = h->field("position");
     if (!pos || !pos->hot) return fail("compute_particle_bins needs a per-particle position field");
@@ -549,8 +561,17 @@ int run_kernel(psdata_hip *h, const std::string &internal) {
     sphk::FieldPtrs f = field_ptrs(h);
     hipStream_t s = h->stream;
     int rc = 0;
-    if (internal == "compute_density")
+    const bool listed = h->pairs.list != nullptr;
+    const sphk::PairCache cache = h->pairs;
+    if (internal != "compute_forces_fluids") h->pairs.valid = 0;
+    if (internal == "compute_density" && listed)
+        rc = timed(h, "compute_density",
+                   [&] { return sphk::launch_compute_density_listed(p, f, cache, s); });
+    else if (internal == "compute_density")
         rc = timed(h, "compute_density", [&] { return sphk::launch_compute_density(p, f, s); });
+    else if (internal == "compute_forces_fluids" && listed && cache.valid)
+        rc = timed(h, "compute_forces_fluids",
+                   [&] { return sphk::launch_forces_fluids_listed(p, f, cache, s); });
     else if (internal == "compute_forces_fluids")
         rc = timed(h, "compute_forces_fluids", [&] { return sphk::launch_forces_fluids(p, f, s); });
     else if (internal == "step_forward")
@@ -581,6 +602,7 @@ int run_kernel(psdata_hip *h, const std::string &internal) {
         return fail("unknown kernel '%s'", internal.c_str());
     if (rc) return -1;
     mark_written(h, k);
+    if (internal == "compute_density" && listed) h->pairs.valid = 1;
     return 0;
 }
 
@@ -632,7 +654,8 @@ void release(psdata_hip *h) {
         if (f.buf[1]) hipFree(f.buf[1]);
     }
     unsigned *scratch[] = {h->slot_id[0], h->slot_id[1], h->perm, h->bins.key, h->bins.local, h->bins.tmp_id,
-                           h->bins.tmp_src, h->bins.tmp_key, h->bins.tile_sums};
+                           h->bins.tmp_src, h->bins.tmp_key, h->bins.tile_sums, h->pairs.list, h->pairs.count,
+                           reinterpret_cast<unsigned *>(h->pairs.pressure)};
     for (unsigned *p : scratch)
         if (p) hipFree(p);
     if (h->staging) hipFree(h->staging);
@@ -691,6 +714,7 @@ int download_field(psdata_hip *h, const psdata &data, DevField *f) {
 
 // Upload a set of fields; a change of n first restores particle order.
 int upload_fields(psdata_hip *h, const psdata &data, const std::vector<DevField *> &list) {
+    h->pairs.valid = 0;
     for (DevField *f : list) {
         if (f->name != "n") continue;
         unsigned new_n;
@@ -838,7 +862,11 @@ extern "C" int create_psdata_hip(psdata *data, const char *file_list, psdata_hip
         }
     if (hipMalloc(reinterpret_cast<void **>(&h->bins.tile_sums), tiles * 4) != hipSuccess ||
         hipMalloc(&h->staging, max_particle_bytes) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void **>(&h->sums), (7 * sphk::kReduceBlocks + 8) * sizeof(double)) != hipSuccess) {
+        hipMalloc(reinterpret_cast<void **>(&h->sums), (7 * sphk::kReduceBlocks + 8) * sizeof(double)) != hipSuccess ||
+        (h->kset.fluids && h->has_grid &&
+         (hipMalloc(reinterpret_cast<void **>(&h->pairs.list), pn4 * sphk::kListCap) != hipSuccess ||
+          hipMalloc(reinterpret_cast<void **>(&h->pairs.count), pn4) != hipSuccess ||
+          hipMalloc(reinterpret_cast<void **>(&h->pairs.pressure), pn4) != hipSuccess))) {
         release(h);
         return fail("out of device memory (scratch)");
     }
