@@ -87,8 +87,9 @@ def main():
     import numpy as np
 
     import sph
-    from sph import roofline, scenes
+    from sph import bindings, roofline, scenes
 
+    bindings.set_log_level(3)          # library notes would share stdout with the JSON line
     sph.init_hip()
     scene = scenes.baseline_config(WORKLOADS[args.workload])
     log(f"workload {scene.name}: {scene.n} particles, system {scene.system}")

@@ -20,9 +20,15 @@ struct V3 {
     float x, y, z;
 };
 
+// 12-byte tuple with 4-byte alignment: one global_load_dwordx3 per vector.
+struct __attribute__((packed, aligned(4))) Packed3 {
+    float x, y, z;
+};
+
 __device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
 __device__ __forceinline__ V3 load3(const float *p, unsigned i) {
-    return V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    const Packed3 t = *reinterpret_cast<const Packed3 *>(p + 3 * static_cast<size_t>(i));
+    return V3{t.x, t.y, t.z};
 }
 __device__ __forceinline__ void store3(float *p, unsigned i, V3 v) {
     p[3 * i] = v.x;

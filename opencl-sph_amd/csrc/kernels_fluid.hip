@@ -81,17 +81,36 @@ __device__ __forceinline__ float pressure(const StepParams &p, float rho);
 
 // compute_density for fluid programs, in two phases per lane:
 //   1. scan the candidates with the square-root-free range test and compact
-//      the in-range slots into this lane's column of an LDS buffer;
+//      the in-range slots into this lane's column of an LDS buffer, as 16-bit
+//      entries (row number << 12 | offset from the row's first slot);
 //   2. evaluate the kernel sum over the compacted slots (every lane busy),
 //      writing them out as the pair cache's neighbour list.
 // The summation order is the candidate order, as in the reference.  Lanes
-// near a mirror plane, or with more than kListCap in-range candidates, take
-// the reference loop instead.
+// near a mirror plane, with more than kListCap in-range candidates or with a
+// row longer than 4096 slots take the reference loop instead.
 constexpr unsigned kListBlock = 128;
+constexpr unsigned kRowBits = 12;
+
+// First slot of neighbour row r (0..8, (dz, dy) = (r / 3 - 1, r % 3 - 1))
+// around the cell (xo, yo, zo), and one past its last slot; false when the
+// row lies outside the grid.
+__device__ __forceinline__ bool row_range(const GridGeom &g, unsigned n, unsigned xo, unsigned yo, unsigned zo,
+                                          unsigned r, const unsigned *__restrict__ celloffset,
+                                          const unsigned *__restrict__ gridcount, unsigned *b, unsigned *e) {
+    const int zn = static_cast<int>(zo) + static_cast<int>(r / 3u) - 1;
+    const int yn = static_cast<int>(yo) + static_cast<int>(r % 3u) - 1;
+    if (zn < 0 || zn >= static_cast<int>(g.rz) || yn < 0 || yn >= static_cast<int>(g.ry)) return false;
+    const unsigned x0 = xo > 0u ? xo - 1u : 0u;
+    const unsigned x1 = xo + 1u < g.rx ? xo + 1u : xo;
+    const unsigned row = static_cast<unsigned>(zn) * g.layer + static_cast<unsigned>(yn) * g.rx;
+    *b = min(celloffset[row + x0], n);
+    *e = min(celloffset[row + x1] + gridcount[row + x1], n);
+    return true;
+}
 
 template <bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_listed(StepParams p, FieldPtrs f, PairCache c) {
-    __shared__ unsigned buf[kListCap * kListBlock];
+    __shared__ unsigned short buf[kListCap * kListBlock];
     const unsigned i = blockIdx.x * kListBlock + threadIdx.x;
     if (i >= p.n) return;
     const KernelConsts kc = kconsts(p);
@@ -99,25 +118,46 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(StepParams p, Fie
     const V3 xi = load3(pos, i);
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned key = f.gridcell[i];
-    unsigned *col = buf + threadIdx.x;
+    unsigned short *col = buf + threadIdx.x;
     unsigned cnt = 0;
-    if (!inear) {
-        for_each_candidate(p.grid, p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
-            const float s = sqlength(load3(pos, j) - xi);
-            if (!(s >= p.r2_out)) {
-                if (cnt < kListCap) col[cnt * kListBlock] = j;
-                ++cnt;
+    bool fast = !inear && key < p.grid.ncells;
+    unsigned xo = 0, yo = 0, zo = 0;
+    if (fast) {
+        zo = key / p.grid.layer;
+        const unsigned rem = key - p.grid.layer * zo;
+        yo = rem / p.grid.rx;
+        xo = rem - p.grid.rx * yo;
+#pragma unroll 1
+        for (unsigned r = 0; r < 9u; ++r) {
+            unsigned b, e;
+            if (!row_range(p.grid, p.n, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e)) continue;
+            if (e - b > (1u << kRowBits)) fast = false;
+            for (unsigned j = b; j < e; ++j) {
+                const float s = sqlength(load3(pos, j) - xi);
+                if (!(s >= p.r2_out)) {
+                    if (cnt < kListCap)
+                        col[cnt * kListBlock] = static_cast<unsigned short>((r << kRowBits) | (j - b));
+                    ++cnt;
+                }
             }
-        });
+        }
+        if (cnt > kListCap) fast = false;
     }
     Acc<KAHAN> rho;
     unsigned nn = 0;
-    if (!inear && cnt <= kListCap) {
+    if (fast) {
+        unsigned cur = 0xffu, base = 0, e = 0;
         for (unsigned k = 0; k < cnt; ++k) {
-            const unsigned j = col[k * kListBlock];
-            const float r = length(load3(pos, j) - xi);
-            rho.add(w_poly6(r, kc));
-            if (r < kc.h) ++nn;
+            const unsigned ent = col[k * kListBlock];
+            const unsigned r = ent >> kRowBits;
+            if (r != cur) {
+                cur = r;
+                row_range(p.grid, p.n, xo, yo, zo, r, f.celloffset, f.gridcount, &base, &e);
+            }
+            const unsigned j = base + (ent & ((1u << kRowBits) - 1u));
+            const float rr = length(load3(pos, j) - xi);
+            rho.add(w_poly6(rr, kc));
+            if (rr < kc.h) ++nn;
             c.list[static_cast<size_t>(k) * p.n + i] = j;
         }
     } else {
@@ -233,13 +273,13 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(StepParams p, F
     const float rhoi = rho[i];
     const float pi = prs[i];
     Acc<KAHAN> fx, fy, fz;
-    auto pair = [&](unsigned j) {
-        const V3 d = load3(pos, j) - xi;
-        const float r = length(d);
-        if (r > kc.h) return;
-        const V3 rv = load3(vel, j) - vi;
+    auto pair_in_range = [&](unsigned j) {
+        const V3 xj = load3(pos, j), vj = load3(vel, j);
         const float rhoj = rho[j];
         const float pj = prs[j];
+        const V3 d = xj - xi;
+        const float r = length(d);
+        const V3 rv = vj - vi;
         const float din = kc.h - r;
         const float didj = rhoi * rhoj;
         const float vterm = (p.viscosity * din) * kc.lap;
@@ -250,11 +290,19 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(StepParams p, F
         fy.add((d.y * pterm + rv.y * vterm) / didj);
         fz.add((d.z * pterm + rv.z * vterm) / didj);
     };
+    auto pair = [&](unsigned j) {
+        if (length(load3(pos, j) - xi) > kc.h) return;
+        pair_in_range(j);
+    };
     const unsigned cnt = c.count[i];
     if (cnt <= kListCap) {
+        // List entries are in range by construction (same positions, same
+        // distance expression), so the pair terms need no range test here.
+        unsigned j = cnt ? c.list[i] : 0u;
         for (unsigned k = 0; k < cnt; ++k) {
-            const unsigned j = c.list[static_cast<size_t>(k) * p.n + i];
-            if (j != i) pair(j);
+            const unsigned jn = k + 1 < cnt ? c.list[static_cast<size_t>(k + 1) * p.n + i] : 0u;
+            if (j != i) pair_in_range(j);
+            j = jn;
         }
     } else {
         for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {

@@ -26,9 +26,11 @@ __device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned n
     const unsigned xo = rem - g.rx * yo;
     const unsigned x0 = xo > 0u ? xo - 1u : 0u;
     const unsigned x1 = xo + 1u < g.rx ? xo + 1u : xo;
+#pragma unroll 1
     for (int dz = -1; dz <= 1; ++dz) {
         const int zn = static_cast<int>(zo) + dz;
         if (zn < 0 || zn >= static_cast<int>(g.rz)) continue;
+#pragma unroll 1
         for (int dy = -1; dy <= 1; ++dy) {
             const int yn = static_cast<int>(yo) + dy;
             if (yn < 0 || yn >= static_cast<int>(g.ry)) continue;
