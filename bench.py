@@ -43,6 +43,27 @@ def grid_pair_counts(system, scene):
     return candidates, in_range
 
 
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB (a copy kernel of known size):
+    profiled beside the workload, its FETCH_SIZE / WRITE_SIZE readings give
+    the scale that converts those counters to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed in the copy probe")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    for _ in range(3):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: 3 x {mib} MiB device-to-device")
+
+
 def cpu_baseline(scene, budget_s):
     """The FP32 oracle on the host cores, full scene, as many whole steps as
     fit the time budget (at least one after a warm-up step)."""
@@ -76,8 +97,11 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dam_break_4m", choices=sorted(WORKLOADS))
-    ap.add_argument("--plain-sums", action="store_true", help="plain FP32 neighbour sums (default: Kahan)")
+    ap.add_argument("--sums", default="kahan-density", choices=["plain", "kahan-density", "kahan-all"],
+                    help="neighbour-sum precision (default: Kahan-compensated density sums)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work (0: skip)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the engine runs one scene set per device and this build has no "
@@ -91,11 +115,14 @@ def main():
 
     bindings.set_log_level(3)          # library notes would share stdout with the JSON line
     sph.init_hip()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
     scene = scenes.baseline_config(WORKLOADS[args.workload])
     log(f"workload {scene.name}: {scene.n} particles, system {scene.system}")
     system = scenes.build(scene)
     handle = system.psdata_opencl
-    handle.set_precision(compensated_sums=not args.plain_sums, polar_fp64=True)
+    handle.set_precision(compensated_sums={"plain": 0, "kahan-density": 1, "kahan-all": 3}[args.sums],
+                         polar_fp64=True)
 
     system.run(args.warmup)
     handle.synchronize()
@@ -155,7 +182,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded lattice with uniform jitter)",
         "config": {"workload": scene.name, "particles": scene.n, "cells": cells, "system": scene.system,
-                   "path": "plain", "sums": "plain" if args.plain_sums else "kahan",
+                   "path": "plain", "sums": args.sums,
                    "fluid_model": "tait eos, mass-consistent force" if scene.system == "fluid" else None},
         "roofline": roof, "cpu_baseline": cpu,
     }

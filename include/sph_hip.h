@@ -165,8 +165,9 @@ int step_device_hip(psdata_hip *pso, const char *system, unsigned int num_steps,
 /* Block until all work queued on the handle's stream has finished. */
 int synchronize_hip(psdata_hip *pso);
 
-/* Precision of neighbour sums: 0 = plain FP32 in reference order (default),
- * 1 = Kahan-compensated FP32.  polar_fp64 != 0 computes the corotational
+/* Precision of neighbour sums, a bit mask: 0 = plain FP32 in reference order
+ * (default); bit 0 (1) = Kahan-compensated density sums; bit 1 (2) =
+ * Kahan-compensated force / deformation sums.  polar_fp64 != 0 computes the corotational
  * polar decomposition and the SVK inverse in FP64 (default 1). */
 int set_precision_hip(psdata_hip *pso, int compensated_sums, int polar_fp64);
 

@@ -11,6 +11,10 @@ namespace sphk {
 constexpr int kMaxSymmetryPlanes = 5;      // 2^5 plane combinations fit one mask word
 constexpr int kMaxPlaneConstraints = 32;   // bit c of plane_constraints_particles
 
+// Kahan-compensated neighbour sums, per family of passes.
+constexpr int kKahanDensity = 1;           // density, rest density
+constexpr int kKahanForces = 2;            // forces, A_pq / A_qq, displacement gradient
+
 // Uniform grid: res and the box [lo, hi] per axis.
 struct GridGeom {
     unsigned rx, ry, rz, layer, ncells;
@@ -38,7 +42,7 @@ struct StepParams {
     unsigned body_count;                           // 0: no body fields
     int eos_tait;                                  // conf field eos_tait: Tait EOS (exponent 7)
     int mass_consistent;                           // conf field mass_consistent: fluid force times m
-    int compensated;                               // Kahan-compensated neighbour sums
+    int compensated;                               // kKahanDensity | kKahanForces
     int polar_fp64;                                // FP64 polar decomposition / inverse
 };
 

@@ -40,7 +40,7 @@ struct Acc<true> {
 // ORIGINAL = compute_original_density (density0, no neighbour count).
 template <bool ORIGINAL, bool KAHAN>
 __global__ void __launch_bounds__(kBlock) k_density(StepParams p, FieldPtrs f) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = xcd_slot(blockDim.x);
     if (i >= p.n) return;
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
@@ -111,7 +111,7 @@ __device__ __forceinline__ bool row_range(const GridGeom &g, unsigned n, unsigne
 template <bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_listed(StepParams p, FieldPtrs f, PairCache c) {
     __shared__ unsigned short buf[kListCap * kListBlock];
-    const unsigned i = blockIdx.x * kListBlock + threadIdx.x;
+    const unsigned i = xcd_slot(kListBlock);
     if (i >= p.n) return;
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
@@ -217,7 +217,7 @@ __device__ __forceinline__ float cube_wall(float x, float fc) {
 
 template <bool KAHAN>
 __global__ void __launch_bounds__(kBlock) k_forces_fluids(StepParams p, FieldPtrs f) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = xcd_slot(blockDim.x);
     if (i >= p.n) return;
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(StepParams p, FieldPtr
 // equation of state per pair).  Overflowed lists fall back to the scan.
 template <bool KAHAN>
 __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(StepParams p, FieldPtrs f, PairCache c) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = xcd_slot(blockDim.x);
     if (i >= p.n) return;
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
@@ -401,23 +401,25 @@ __global__ void k_sums_final(const double *partials, double *out7, unsigned n, f
 }
 
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
+// Neighbour passes: a multiple of 8 blocks for the XCD-aware slot mapping.
+inline dim3 nbr_grid(unsigned n, unsigned block) { return dim3(xcd_grid((n + block - 1) / block)); }
 
 }  // namespace
 
 hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    if (p.compensated)
-        hipLaunchKernelGGL((k_density<false, true>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    if (p.compensated & kKahanDensity)
+        hipLaunchKernelGGL((k_density<false, true>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
     else
-        hipLaunchKernelGGL((k_density<false, false>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+        hipLaunchKernelGGL((k_density<false, false>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
     return hipGetLastError();
 }
 
 hipError_t launch_compute_density_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
                                          hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    const dim3 grid((p.n + kListBlock - 1) / kListBlock), block(kListBlock);
-    if (p.compensated)
+    const dim3 grid = nbr_grid(p.n, kListBlock), block(kListBlock);
+    if (p.compensated & kKahanDensity)
         hipLaunchKernelGGL((k_density_listed<true>), grid, block, 0, s, p, f, c);
     else
         hipLaunchKernelGGL((k_density_listed<false>), grid, block, 0, s, p, f, c);
@@ -427,28 +429,28 @@ hipError_t launch_compute_density_listed(const StepParams &p, const FieldPtrs &f
 hipError_t launch_forces_fluids_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
                                        hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    if (p.compensated)
-        hipLaunchKernelGGL((k_forces_fluids_listed<true>), grid_for(p.n), dim3(kBlock), 0, s, p, f, c);
+    if (p.compensated & kKahanForces)
+        hipLaunchKernelGGL((k_forces_fluids_listed<true>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f, c);
     else
-        hipLaunchKernelGGL((k_forces_fluids_listed<false>), grid_for(p.n), dim3(kBlock), 0, s, p, f, c);
+        hipLaunchKernelGGL((k_forces_fluids_listed<false>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f, c);
     return hipGetLastError();
 }
 
 hipError_t launch_compute_original_density(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    if (p.compensated)
-        hipLaunchKernelGGL((k_density<true, true>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    if (p.compensated & kKahanDensity)
+        hipLaunchKernelGGL((k_density<true, true>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
     else
-        hipLaunchKernelGGL((k_density<true, false>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+        hipLaunchKernelGGL((k_density<true, false>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
     return hipGetLastError();
 }
 
 hipError_t launch_forces_fluids(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    if (p.compensated)
-        hipLaunchKernelGGL((k_forces_fluids<true>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    if (p.compensated & kKahanForces)
+        hipLaunchKernelGGL((k_forces_fluids<true>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
     else
-        hipLaunchKernelGGL((k_forces_fluids<false>), grid_for(p.n), dim3(kBlock), 0, s, p, f);
+        hipLaunchKernelGGL((k_forces_fluids<false>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
     return hipGetLastError();
 }
 

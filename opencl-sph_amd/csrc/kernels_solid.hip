@@ -112,7 +112,7 @@ __device__ __forceinline__ bool in_body(const FieldPtrs &f, unsigned i) {
 
 template <bool KAHAN, bool FP64>
 __global__ void __launch_bounds__(kBlock) k_rotations_strains(StepParams p, FieldPtrs f) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = xcd_slot(blockDim.x);
     if (i >= p.n || in_body(f, i)) return;
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(kBlock) k_stresses_corot(StepParams p, FieldPt
 
 template <bool KAHAN, bool FP64>
 __global__ void __launch_bounds__(kBlock) k_deformations_svk(StepParams p, FieldPtrs f) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = xcd_slot(blockDim.x);
     if (i >= p.n) return;
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(kBlock) k_stresses_svk(StepParams p, FieldPtrs
 
 template <bool CORO, bool KAHAN>
 __global__ void __launch_bounds__(kBlock) k_forces_solids(StepParams p, FieldPtrs f) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = xcd_slot(blockDim.x);
     if (i >= p.n) return;
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
@@ -380,13 +380,14 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(StepParams p, FieldPtr
 }
 
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
+inline dim3 nbr_grid(unsigned n) { return dim3(xcd_grid((n + kBlock - 1) / kBlock)); }
 
 }  // namespace
 
 hipError_t launch_rotations_and_strains(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    dim3 g = grid_for(p.n), b(kBlock);
-    if (p.compensated) {
+    dim3 g = nbr_grid(p.n), b(kBlock);
+    if (p.compensated & kKahanForces) {
         if (p.polar_fp64) hipLaunchKernelGGL((k_rotations_strains<true, true>), g, b, 0, s, p, f);
         else hipLaunchKernelGGL((k_rotations_strains<true, false>), g, b, 0, s, p, f);
     } else {
@@ -404,8 +405,8 @@ hipError_t launch_stresses_corotational(const StepParams &p, const FieldPtrs &f,
 
 hipError_t launch_deformations_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    dim3 g = grid_for(p.n), b(kBlock);
-    if (p.compensated) {
+    dim3 g = nbr_grid(p.n), b(kBlock);
+    if (p.compensated & kKahanForces) {
         if (p.polar_fp64) hipLaunchKernelGGL((k_deformations_svk<true, true>), g, b, 0, s, p, f);
         else hipLaunchKernelGGL((k_deformations_svk<true, false>), g, b, 0, s, p, f);
     } else {
@@ -423,12 +424,12 @@ hipError_t launch_stresses_svk(const StepParams &p, const FieldPtrs &f, hipStrea
 
 hipError_t launch_forces_solids(const StepParams &p, const FieldPtrs &f, bool corotational, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    dim3 g = grid_for(p.n), b(kBlock);
+    dim3 g = nbr_grid(p.n), b(kBlock);
     if (corotational) {
-        if (p.compensated) hipLaunchKernelGGL((k_forces_solids<true, true>), g, b, 0, s, p, f);
+        if (p.compensated & kKahanForces) hipLaunchKernelGGL((k_forces_solids<true, true>), g, b, 0, s, p, f);
         else hipLaunchKernelGGL((k_forces_solids<true, false>), g, b, 0, s, p, f);
     } else {
-        if (p.compensated) hipLaunchKernelGGL((k_forces_solids<false, true>), g, b, 0, s, p, f);
+        if (p.compensated & kKahanForces) hipLaunchKernelGGL((k_forces_solids<false, true>), g, b, 0, s, p, f);
         else hipLaunchKernelGGL((k_forces_solids<false, false>), g, b, 0, s, p, f);
     }
     return hipGetLastError();

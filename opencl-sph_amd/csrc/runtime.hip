@@ -1015,7 +1015,7 @@ extern "C" int synchronize_hip(psdata_hip *h) {
 
 extern "C" int set_precision_hip(psdata_hip *h, int compensated_sums, int polar_fp64) {
     if (!h) return fail("null handle");
-    h->compensated = compensated_sums ? 1 : 0;
+    h->compensated = compensated_sums & (sphk::kKahanDensity | sphk::kKahanForces);
     h->polar_fp64 = polar_fp64 ? 1 : 0;
     return 0;
 }

@@ -195,8 +195,12 @@ class PsdataHip:
     def synchronize(self):
         check(_synchronize(self.handle))
 
-    def set_precision(self, compensated_sums=False, polar_fp64=True):
-        check(_set_precision(self.handle, int(bool(compensated_sums)), int(bool(polar_fp64))))
+    KAHAN_DENSITY, KAHAN_FORCES = 1, 2
+
+    def set_precision(self, compensated_sums=0, polar_fp64=True):
+        """compensated_sums: bit mask of KAHAN_DENSITY / KAHAN_FORCES (True = both)."""
+        flags = 3 if compensated_sums is True else int(compensated_sums)
+        check(_set_precision(self.handle, flags, int(bool(polar_fp64))))
 
     def set_profiling(self, enabled=True):
         check(_set_profiling(self.handle, int(bool(enabled))))
