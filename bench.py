@@ -43,6 +43,27 @@ def grid_pair_counts(system, scene):
     return candidates, in_range
 
 
+# Timed kernel name -> device symbol prefix in the rocprofv3 counter summaries.
+KERNEL_SYMBOLS = {"compute_density": "k_density_listed", "compute_forces_fluids": "k_forces_fluids_listed",
+                  "bin_rank_place": "k_rank_place", "step_forward": "k_step_forward",
+                  "bin_scatter": "k_scatter_unsorted", "bin_key_count": "k_key_count"}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the calibrated FETCH_SIZE /
+    WRITE_SIZE summary committed under profiles/, or None."""
+    path = os.path.join(REPO, "profiles", f"{workload}_hbm_traffic.json")
+    prefix = KERNEL_SYMBOLS.get(kernel)
+    if prefix is None or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        summary = json.load(f)
+    for name, row in summary.get("kernels", {}).items():
+        if name.startswith(prefix):
+            return int(row["read_bytes"] + row["write_bytes"])
+    return None
+
+
 def copy_probe(mib):
     """One device-to-device copy of `mib` MiB (a copy kernel of known size):
     profiled beside the workload, its FETCH_SIZE / WRITE_SIZE readings give
@@ -154,7 +175,8 @@ def main():
     if bytes_launch is not None:
         achieved = bytes_launch / avg_s / 1e9
         roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.MI355X_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / roofline.MI355X_HBM_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / roofline.MI355X_HBM_GBS, 4),
+                "traffic": measured_traffic(scene.name, dominant),
                 "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_s * 1e3, 5)}
     extra = {}
     if scene.system == "fluid":

@@ -184,6 +184,51 @@ int get_kernel_stats_hip(psdata_hip *pso, int capacity, const char **names,
  * (3), total kinetic energy, and the centre of mass (3).  out has 7 doubles. */
 int global_sums_hip(psdata_hip *pso, double *out);
 
+/* ---- scene batch --------------------------------------------------------
+ * No counterpart in the reference, which runs one psdata per OpenCL program
+ * and steps it with one launch per kernel (src/opencl/particle_system_host.c:
+ * 361-528, scripts/sph/systems.py:148-189).  A batch keeps many independent
+ * scenes resident in one pool of particle slots and grid cells and steps them
+ * all with one launch per pass; each scene advances one step per batch step
+ * until its own step count is used up.  A scene's results are the same as
+ * those of the single-scene path at the same precision settings.  Scenes are
+ * added from a psdata built from their conf (fields as set from Python) and
+ * the kernel file list that create_psdata_hip would take; the system is
+ * fluid (fluids), corotational (solid_corotation) or svk (solid_svk).  All
+ * calls return 0 on success and -1 on error (message: sph_hip_last_error),
+ * except where stated. */
+typedef struct sph_batch sph_batch;
+
+/* Device bytes of a pool of `slots` particle slots, `cells` grid cells and
+ * `scenes` scene entries (what sph_batch_create allocates). */
+size_t sph_batch_device_bytes(unsigned slots, unsigned cells, unsigned scenes);
+/* compensated_sums: bit mask as in set_precision_hip, for every scene. */
+int sph_batch_create(unsigned slots, unsigned cells, unsigned scenes, int compensated_sums, int polar_fp64,
+                     sph_batch **out);
+int sph_batch_free(sph_batch *batch);
+/* Admits a scene that will run `steps` steps.  Returns 1 (not an error) when
+ * the pool has no room for it now; *scene receives its handle.  Solid scenes
+ * compute their reference configuration (init_original_position,
+ * compute_original_density) before their first step. */
+int sph_batch_add(sph_batch *batch, psdata *data, const char *kernel_file_list, unsigned steps, int *scene);
+/* Releases a scene's slots and cells (finished or not). */
+int sph_batch_remove(sph_batch *batch, int scene);
+/* Enqueues num_steps batch steps (asynchronous; sph_batch_synchronize or a
+ * read waits). */
+int sph_batch_step(sph_batch *batch, unsigned num_steps);
+/* Capture runs of `steps_per_graph` batch steps (even) into a hipGraph and
+ * replay it while the pool layout is unchanged; 0 launches every pass. */
+int sph_batch_set_graph_steps(sph_batch *batch, unsigned steps_per_graph);
+int sph_batch_synchronize(sph_batch *batch);
+int sph_batch_steps_left(sph_batch *batch, int scene, unsigned *steps_left);
+/* Copies fields of a scene, in particle order, into the matching fields of
+ * `data` (position, velocity, force, density, density0, originalpos,
+ * rotation, strain, stress, deformation, body_number). */
+int sph_batch_read(sph_batch *batch, int scene, psdata data, size_t num_fields, const char *const *names);
+/* out[0..7]: resident scenes, scenes with steps left, used slots, used
+ * cells, slot capacity, cell capacity, graph captures, graph launches. */
+int sph_batch_info(sph_batch *batch, unsigned long long *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,7 @@ SOURCES = [
     "kernels_fluid.hip",
     "kernels_solid.hip",
     "runtime.hip",
+    "batch.hip",
 ]
 
 ARCH = os.environ.get("SPH_HIP_ARCH", "gfx950")

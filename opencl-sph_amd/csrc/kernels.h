@@ -15,15 +15,26 @@ constexpr int kMaxPlaneConstraints = 32;   // bit c of plane_constraints_particl
 constexpr int kKahanDensity = 1;           // density, rest density
 constexpr int kKahanForces = 2;            // forces, A_pq / A_qq, displacement gradient
 
-// Uniform grid: res and the box [lo, hi] per axis.
+// Uniform grid: res and the box [lo, hi] per axis.  Cell keys are global:
+// the scene's cells are [c0, c0 + ncells) of the cell arrays (c0 = 0 for a
+// single scene).
 struct GridGeom {
     unsigned rx, ry, rz, layer, ncells;
+    unsigned c0;
     float lo[3], hi[3];
 };
 
+enum SystemKind : unsigned { kFluid = 0, kCorotational = 1, kSvk = 2 };
+constexpr unsigned kSceneSetup = 1u;
+
 // Everything the per-particle passes read besides the per-particle fields.
+// A single psdata is one scene at slots [0, n); a batch holds one StepParams
+// per scene, at slots [p0, p0 + n).
 struct StepParams {
-    unsigned n;               // active particles (slots [0, n))
+    unsigned n;               // active particles
+    unsigned p0;              // first slot
+    unsigned system;          // SystemKind (batches dispatch passes on it)
+    unsigned flags;           // kSceneSetup: reference configuration still to compute
     GridGeom grid;
     float h, h2, poly6, spiky, lap;
     // Smallest float s with RN(sqrt(s)) > h: for s = squared distance,
@@ -40,6 +51,7 @@ struct StepParams {
     unsigned nconstraints;                         // plane constraints
     float constraints[kMaxPlaneConstraints][6];
     unsigned body_count;                           // 0: no body fields
+    unsigned body_base;                            // first entry of body_dynamic for this scene
     int eos_tait;                                  // conf field eos_tait: Tait EOS (exponent 7)
     int mass_consistent;                           // conf field mass_consistent: fluid force times m
     int compensated;                               // kKahanDensity | kKahanForces
@@ -100,15 +112,27 @@ hipError_t launch_scatter_words(const void *src, void *dst, const unsigned *map,
 
 // ---- per-particle passes (kernels_fluid.hip, kernels_solid.hip) ------------
 // In-range neighbour lists written by the density pass: for slot i, entries
-// k < count[i] at list[k * n + i] are the slots j with length(x_j - x_i) <= h
+// k < count[i] at list[k * stride + i] are the slots j with length(x_j - x_i) <= h
 // in the reference's visiting order (self included).  count[i] > kListCap
 // means the list overflowed and readers must scan the candidates instead.
 constexpr unsigned kListCap = 48;
 struct PairCache {
-    unsigned *list;        // [kListCap * pnum]
-    unsigned *count;       // [pnum]
-    float *pressure;       // [pnum] equation of state of each slot's density
+    unsigned *list;        // [kListCap * stride]
+    unsigned *count;       // [stride]
+    float *pressure;       // [stride] equation of state of each slot's density
+    unsigned stride;       // slots per list column (pnum, or the batch capacity)
     int valid;             // written by the density pass of the current state
+};
+
+// Per-scene parameters of a batch: slot s belongs to scene scene_of[s]
+// (kNoScene for unused slots) and is active when s < p0 + n of that scene
+// and the scene has steps left.
+constexpr unsigned kNoScene = 0xffffffffu;
+struct BatchTable {
+    const StepParams *scenes;
+    const unsigned *scene_of;
+    const unsigned *steps_left;   // per scene
+    unsigned capacity;            // slots a launch covers (the pool's used slots)
 };
 
 hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipStream_t s);
@@ -128,6 +152,38 @@ hipError_t launch_deformations_svk(const StepParams &p, const FieldPtrs &f, hipS
 hipError_t launch_stresses_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_forces_solids(const StepParams &p, const FieldPtrs &f, bool corotational,
                                 hipStream_t s);
+
+// ---- batches: the same kernels with per-scene parameters ------------------
+// A pass runs for the scenes whose system is in system_mask (bit SystemKind)
+// and whose flags contain all of flag_mask.
+struct SceneFilter {
+    unsigned system_mask;
+    unsigned flag_mask;
+};
+constexpr SceneFilter kAllScenes = {7u, 0u};
+
+// `compensated` is the batch's precision setting (kKahanDensity |
+// kKahanForces), shared by all its scenes.
+hipError_t launch_batch_density_listed(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                       const PairCache &c, int compensated, hipStream_t s);
+hipError_t launch_batch_density(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, int compensated,
+                                hipStream_t s);
+hipError_t launch_batch_original_density(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                         int compensated, hipStream_t s);
+hipError_t launch_batch_init_original_position(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                               hipStream_t s);
+hipError_t launch_batch_forces_fluids_listed(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                             const PairCache &c, int compensated, hipStream_t s);
+hipError_t launch_batch_step_forward(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_batch_rotations_and_strains(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                              int compensated, int polar_fp64, hipStream_t s);
+hipError_t launch_batch_stresses_corotational(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                              hipStream_t s);
+hipError_t launch_batch_deformations_svk(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                         int compensated, int polar_fp64, hipStream_t s);
+hipError_t launch_batch_stresses_svk(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_batch_forces_solids(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                      bool corotational, int compensated, hipStream_t s);
 
 // FP64 reductions over slots [0, n): momentum (3), kinetic energy, centre of
 // mass (3).  `partials` holds 7 * kReduceBlocks doubles.

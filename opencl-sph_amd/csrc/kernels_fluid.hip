@@ -9,7 +9,9 @@
 //
 // One thread per active slot; every sum runs over the candidates in the
 // reference's visiting order, so with compensated sums off each output is
-// the same IEEE operation sequence as the sequential oracle.
+// the same IEEE operation sequence as the sequential oracle.  Every kernel is
+// a template over its parameter source (one scene, or a batch's scene
+// table), so a scene computes the same bits alone and in a batch.
 #include "neighbours.h"
 
 #pragma clang fp contract(off)
@@ -37,38 +39,69 @@ struct Acc<true> {
     __device__ __forceinline__ float value() const { return s; }
 };
 
-// ORIGINAL = compute_original_density (density0, no neighbour count).
-template <bool ORIGINAL, bool KAHAN>
-__global__ void __launch_bounds__(kBlock) k_density(StepParams p, FieldPtrs f) {
-    const unsigned i = xcd_slot(blockDim.x);
-    if (i >= p.n) return;
+__device__ __forceinline__ unsigned linear_slot() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// Equation of state: the reference's linear p = k (rho - rho0)
+// (kernels/fluids.cl:28-29), or Tait with exponent 7 when the conf sets
+// eos_tait.
+__device__ __forceinline__ float pressure(const StepParams &p, float rho) {
+    if (!p.eos_tait) return (rho - p.restdens) * p.stiffness;
+    const float r = rho / p.restdens;
+    const float r2 = r * r, r4 = r2 * r2;
+    return p.stiffness * (r4 * r2 * r - 1.0f);
+}
+
+// The reference density loop: every candidate, direct term then mirror
+// images (kernels/common.cl:634-687).  Returns the kernel sum; counts r < h
+// in *nn (COUNT); when `list` is given, records the slots with r <= h in
+// list[k * stride + i] (at most kListCap) and their number in *listed.
+template <bool KAHAN, bool COUNT>
+__device__ __forceinline__ float density_scan(const StepParams &p, const FieldPtrs &f, unsigned i, V3 xi,
+                                              unsigned inear, unsigned *nn, unsigned *list, unsigned stride,
+                                              unsigned *listed) {
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
-    const V3 xi = load3(pos, i);
-    const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned ncombo = 1u << p.nplanes;
     Acc<KAHAN> rho;
-    unsigned nn = 0;
-    for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+    unsigned cnt = 0, k = 0;
+    for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j);
         const float r = length(xj - xi);
-        if (!(r > kc.h)) rho.add(w_poly6(r, kc));
-        if (!ORIGINAL && r < kc.h) ++nn;
+        if (!(r > kc.h)) {
+            rho.add(w_poly6(r, kc));
+            if (list && k < kListCap) list[static_cast<size_t>(k) * stride + i] = j;
+            ++k;
+        }
+        if (COUNT && r < kc.h) ++cnt;
         if (inear) {
             const unsigned ok = planes_near_j(p, xj, inear);
-            if (ok) {
+            if (ok)
                 for (unsigned b = 1; b < ncombo; ++b) {
                     if (b & ~ok) continue;
                     const float rr = length(reflect_combo(p, b, xj) - xi);
                     if (rr < kc.h) {
                         rho.add(w_poly6(rr, kc));
-                        if (!ORIGINAL) ++nn;
+                        if (COUNT) ++cnt;
                     }
                 }
-            }
         }
     });
-    const float out = rho.value() * p.mass;
+    *nn = cnt;
+    if (listed) *listed = k;
+    return rho.value();
+}
+
+// ORIGINAL = compute_original_density (density0, no neighbour count).
+template <class PS, bool ORIGINAL, bool KAHAN>
+__global__ void __launch_bounds__(kBlock) k_density(PS ps, FieldPtrs f) {
+    const unsigned i = xcd_slot(kBlock);
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
+    const V3 xi = load3(f.position, i);
+    const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
+    unsigned nn = 0;
+    const float rho = density_scan<KAHAN, !ORIGINAL>(p, f, i, xi, inear, &nn, nullptr, 0, nullptr);
+    const float out = rho * p.mass;
     if (ORIGINAL) {
         f.density0[i] = out;
     } else {
@@ -76,8 +109,6 @@ __global__ void __launch_bounds__(kBlock) k_density(StepParams p, FieldPtrs f) {
         if (f.num_neighbours) f.num_neighbours[i] = nn;
     }
 }
-
-__device__ __forceinline__ float pressure(const StepParams &p, float rho);
 
 // compute_density for fluid programs, in two phases per lane:
 //   1. scan the candidates with the square-root-free range test and compact
@@ -92,49 +123,55 @@ constexpr unsigned kListBlock = 128;
 constexpr unsigned kRowBits = 12;
 
 // First slot of neighbour row r (0..8, (dz, dy) = (r / 3 - 1, r % 3 - 1))
-// around the cell (xo, yo, zo), and one past its last slot; false when the
-// row lies outside the grid.
-__device__ __forceinline__ bool row_range(const GridGeom &g, unsigned n, unsigned xo, unsigned yo, unsigned zo,
-                                          unsigned r, const unsigned *__restrict__ celloffset,
+// around the cell (xo, yo, zo) of the scene's grid, and one past its last
+// slot, both clamped to the scene's slots [lo, hi); false when the row lies
+// outside the grid.
+__device__ __forceinline__ bool row_range(const GridGeom &g, unsigned lo, unsigned hi, unsigned xo, unsigned yo,
+                                          unsigned zo, unsigned r, const unsigned *__restrict__ celloffset,
                                           const unsigned *__restrict__ gridcount, unsigned *b, unsigned *e) {
     const int zn = static_cast<int>(zo) + static_cast<int>(r / 3u) - 1;
     const int yn = static_cast<int>(yo) + static_cast<int>(r % 3u) - 1;
     if (zn < 0 || zn >= static_cast<int>(g.rz) || yn < 0 || yn >= static_cast<int>(g.ry)) return false;
     const unsigned x0 = xo > 0u ? xo - 1u : 0u;
     const unsigned x1 = xo + 1u < g.rx ? xo + 1u : xo;
-    const unsigned row = static_cast<unsigned>(zn) * g.layer + static_cast<unsigned>(yn) * g.rx;
-    *b = min(celloffset[row + x0], n);
-    *e = min(celloffset[row + x1] + gridcount[row + x1], n);
+    const unsigned row = g.c0 + static_cast<unsigned>(zn) * g.layer + static_cast<unsigned>(yn) * g.rx;
+    *b = min(max(celloffset[row + x0], lo), hi);
+    *e = min(max(celloffset[row + x1] + gridcount[row + x1], lo), hi);
     return true;
 }
 
-template <bool KAHAN>
-__global__ void __launch_bounds__(kListBlock) k_density_listed(StepParams p, FieldPtrs f, PairCache c) {
+template <class PS, bool KAHAN>
+__global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs f, PairCache c) {
     __shared__ unsigned short buf[kListCap * kListBlock];
     const unsigned i = xcd_slot(kListBlock);
-    if (i >= p.n) return;
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
+    const unsigned lo = p.p0, hi = p.p0 + p.n;
+    const float r2_out = p.r2_out;
+    const GridGeom g = p.grid;
     const V3 xi = load3(pos, i);
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned key = f.gridcell[i];
     unsigned short *col = buf + threadIdx.x;
     unsigned cnt = 0;
-    bool fast = !inear && key < p.grid.ncells;
+    bool fast = !inear && key >= g.c0 && key - g.c0 < g.ncells;
     unsigned xo = 0, yo = 0, zo = 0;
     if (fast) {
-        zo = key / p.grid.layer;
-        const unsigned rem = key - p.grid.layer * zo;
-        yo = rem / p.grid.rx;
-        xo = rem - p.grid.rx * yo;
+        const unsigned local = key - g.c0;
+        zo = local / g.layer;
+        const unsigned rem = local - g.layer * zo;
+        yo = rem / g.rx;
+        xo = rem - g.rx * yo;
 #pragma unroll 1
         for (unsigned r = 0; r < 9u; ++r) {
             unsigned b, e;
-            if (!row_range(p.grid, p.n, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e)) continue;
+            if (!row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e)) continue;
             if (e - b > (1u << kRowBits)) fast = false;
             for (unsigned j = b; j < e; ++j) {
                 const float s = sqlength(load3(pos, j) - xi);
-                if (!(s >= p.r2_out)) {
+                if (!(s >= r2_out)) {
                     if (cnt < kListCap)
                         col[cnt * kListBlock] = static_cast<unsigned short>((r << kRowBits) | (j - b));
                     ++cnt;
@@ -143,64 +180,33 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(StepParams p, Fie
         }
         if (cnt > kListCap) fast = false;
     }
-    Acc<KAHAN> rho;
+    float rho;
     unsigned nn = 0;
     if (fast) {
+        Acc<KAHAN> acc;
         unsigned cur = 0xffu, base = 0, e = 0;
         for (unsigned k = 0; k < cnt; ++k) {
             const unsigned ent = col[k * kListBlock];
             const unsigned r = ent >> kRowBits;
             if (r != cur) {
                 cur = r;
-                row_range(p.grid, p.n, xo, yo, zo, r, f.celloffset, f.gridcount, &base, &e);
+                row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &base, &e);
             }
             const unsigned j = base + (ent & ((1u << kRowBits) - 1u));
             const float rr = length(load3(pos, j) - xi);
-            rho.add(w_poly6(rr, kc));
+            acc.add(w_poly6(rr, kc));
             if (rr < kc.h) ++nn;
-            c.list[static_cast<size_t>(k) * p.n + i] = j;
+            c.list[static_cast<size_t>(k) * c.stride + i] = j;
         }
+        rho = acc.value();
     } else {
-        const unsigned ncombo = 1u << p.nplanes;
-        cnt = 0;
-        for_each_candidate(p.grid, p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
-            const V3 xj = load3(pos, j);
-            const float r = length(xj - xi);
-            if (!(r > kc.h)) {
-                rho.add(w_poly6(r, kc));
-                if (cnt < kListCap) c.list[static_cast<size_t>(cnt) * p.n + i] = j;
-                ++cnt;
-            }
-            if (r < kc.h) ++nn;
-            if (inear) {
-                const unsigned ok = planes_near_j(p, xj, inear);
-                if (ok)
-                    for (unsigned b = 1; b < ncombo; ++b) {
-                        if (b & ~ok) continue;
-                        const float rr = length(reflect_combo(p, b, xj) - xi);
-                        if (rr < kc.h) {
-                            rho.add(w_poly6(rr, kc));
-                            ++nn;
-                        }
-                    }
-            }
-        });
+        rho = density_scan<KAHAN, true>(p, f, i, xi, inear, &nn, c.list, c.stride, &cnt);
     }
-    const float out = rho.value() * p.mass;
+    const float out = rho * p.mass;
     f.density[i] = out;
     if (f.num_neighbours) f.num_neighbours[i] = nn;
     c.count[i] = cnt;
     c.pressure[i] = pressure(p, out);
-}
-
-// Equation of state: the reference's linear p = k (rho - rho0)
-// (kernels/fluids.cl:28-29), or Tait with exponent 7 when the conf sets
-// eos_tait.
-__device__ __forceinline__ float pressure(const StepParams &p, float rho) {
-    if (!p.eos_tait) return (rho - p.restdens) * p.stiffness;
-    const float r = rho / p.restdens;
-    const float r2 = r * r, r4 = r2 * r2;
-    return p.stiffness * (r4 * r2 * r - 1.0f);
 }
 
 // Penalty walls at +-2 m, evaluated in double as in the reference macro
@@ -215,10 +221,44 @@ __device__ __forceinline__ float cube_wall(float x, float fc) {
     return fc;
 }
 
+// The pressure + viscosity pair term of compute_forces_fluids
+// (kernels/fluids.cl:27-50), for a pair already known to be in range, and
+// the final scaling and walls (kernels/fluids.cl:56-62).
 template <bool KAHAN>
-__global__ void __launch_bounds__(kBlock) k_forces_fluids(StepParams p, FieldPtrs f) {
-    const unsigned i = xcd_slot(blockDim.x);
-    if (i >= p.n) return;
+struct FluidForce {
+    Acc<KAHAN> fx, fy, fz;
+    __device__ __forceinline__ void pair(const StepParams &p, const KernelConsts &kc, V3 xi, V3 vi, float rhoi,
+                                         float pi, V3 xj, V3 vj, float rhoj, float pj) {
+        const V3 d = xj - xi;
+        const float r = length(d);
+        const V3 rv = vj - vi;
+        const float din = kc.h - r;
+        const float didj = rhoi * rhoj;
+        const float vterm = (p.viscosity * din) * kc.lap;
+        // (p_i + p_j) * 0.5 / r * din * din * spiky, in double
+        // (the 0.5 literal promotes it, kernels/fluids.cl:48)
+        const float pterm = static_cast<float>(static_cast<double>(pi + pj) * 0.5 / static_cast<double>(r) *
+                                               static_cast<double>(din) * static_cast<double>(din) *
+                                               static_cast<double>(kc.spiky));
+        fx.add((d.x * pterm + rv.x * vterm) / didj);
+        fy.add((d.y * pterm + rv.y * vterm) / didj);
+        fz.add((d.z * pterm + rv.z * vterm) / didj);
+    }
+    __device__ __forceinline__ V3 finish(const StepParams &p, V3 xi) const {
+        V3 fi = V3{fx.value(), fy.value(), fz.value()} * p.mass;
+        if (p.mass_consistent) fi = fi * p.mass;   // the reference sums m_j only
+        fi.x = cube_wall(xi.x, fi.x);
+        fi.y = cube_wall(xi.y, fi.y);
+        fi.z = cube_wall(xi.z, fi.z);
+        return fi;
+    }
+};
+
+template <class PS, bool KAHAN>
+__global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
+    const unsigned i = xcd_slot(kBlock);
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
     const float *__restrict__ vel = f.velocity;
@@ -226,44 +266,25 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(StepParams p, FieldPtr
     const V3 xi = load3(pos, i), vi = load3(vel, i);
     const float rhoi = rho[i];
     const float pi = pressure(p, rhoi);
-    Acc<KAHAN> fx, fy, fz;
-    for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+    FluidForce<KAHAN> acc;
+    for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         if (j == i) return;
         const V3 xj = load3(pos, j);
-        const V3 d = xj - xi;
-        const float r = length(d);
-        if (r > kc.h) return;
-        const V3 vj = load3(vel, j);
-        const V3 rv = vj - vi;
+        if (length(xj - xi) > kc.h) return;
         const float rhoj = rho[j];
-        const float pj = pressure(p, rhoj);
-        const float din = kc.h - r;
-        const float didj = rhoi * rhoj;
-        const float vterm = (p.viscosity * din) * kc.lap;
-        // (p_i + p_j) * 0.5 / r * din * din * spiky, in double
-        // (the 0.5 literal promotes it, kernels/fluids.cl:48)
-        const float pterm = static_cast<float>(static_cast<double>(pi + pj) * 0.5 /
-                                               static_cast<double>(r) * static_cast<double>(din) *
-                                               static_cast<double>(din) * static_cast<double>(kc.spiky));
-        fx.add((d.x * pterm + rv.x * vterm) / didj);
-        fy.add((d.y * pterm + rv.y * vterm) / didj);
-        fz.add((d.z * pterm + rv.z * vterm) / didj);
+        acc.pair(p, kc, xi, vi, rhoi, pi, xj, load3(vel, j), rhoj, pressure(p, rhoj));
     });
-    V3 fi = V3{fx.value(), fy.value(), fz.value()} * p.mass;
-    if (p.mass_consistent) fi = fi * p.mass;   // the reference sums m_j only
-    fi.x = cube_wall(xi.x, fi.x);
-    fi.y = cube_wall(xi.y, fi.y);
-    fi.z = cube_wall(xi.z, fi.z);
-    store3(f.force, i, fi);
+    store3(f.force, i, acc.finish(p, xi));
 }
 
 // compute_forces_fluids over the pair cache: the in-range list of each slot
 // and the pressures of the density pass (identical values to evaluating the
 // equation of state per pair).  Overflowed lists fall back to the scan.
-template <bool KAHAN>
-__global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(StepParams p, FieldPtrs f, PairCache c) {
-    const unsigned i = xcd_slot(blockDim.x);
-    if (i >= p.n) return;
+template <class PS, bool KAHAN>
+__global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtrs f, PairCache c) {
+    const unsigned i = xcd_slot(kBlock);
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
     const float *__restrict__ vel = f.velocity;
@@ -272,61 +293,41 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(StepParams p, F
     const V3 xi = load3(pos, i), vi = load3(vel, i);
     const float rhoi = rho[i];
     const float pi = prs[i];
-    Acc<KAHAN> fx, fy, fz;
-    auto pair_in_range = [&](unsigned j) {
-        const V3 xj = load3(pos, j), vj = load3(vel, j);
-        const float rhoj = rho[j];
-        const float pj = prs[j];
-        const V3 d = xj - xi;
-        const float r = length(d);
-        const V3 rv = vj - vi;
-        const float din = kc.h - r;
-        const float didj = rhoi * rhoj;
-        const float vterm = (p.viscosity * din) * kc.lap;
-        const float pterm = static_cast<float>(static_cast<double>(pi + pj) * 0.5 /
-                                               static_cast<double>(r) * static_cast<double>(din) *
-                                               static_cast<double>(din) * static_cast<double>(kc.spiky));
-        fx.add((d.x * pterm + rv.x * vterm) / didj);
-        fy.add((d.y * pterm + rv.y * vterm) / didj);
-        fz.add((d.z * pterm + rv.z * vterm) / didj);
-    };
-    auto pair = [&](unsigned j) {
-        if (length(load3(pos, j) - xi) > kc.h) return;
-        pair_in_range(j);
-    };
+    FluidForce<KAHAN> acc;
     const unsigned cnt = c.count[i];
     if (cnt <= kListCap) {
         // List entries are in range by construction (same positions, same
         // distance expression), so the pair terms need no range test here.
         unsigned j = cnt ? c.list[i] : 0u;
         for (unsigned k = 0; k < cnt; ++k) {
-            const unsigned jn = k + 1 < cnt ? c.list[static_cast<size_t>(k + 1) * p.n + i] : 0u;
-            if (j != i) pair_in_range(j);
+            const unsigned jn = k + 1 < cnt ? c.list[static_cast<size_t>(k + 1) * c.stride + i] : 0u;
+            if (j != i) acc.pair(p, kc, xi, vi, rhoi, pi, load3(pos, j), load3(vel, j), rho[j], prs[j]);
             j = jn;
         }
     } else {
-        for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
-            if (j != i) pair(j);
+        for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+            if (j == i) return;
+            const V3 xj = load3(pos, j);
+            if (length(xj - xi) > kc.h) return;
+            acc.pair(p, kc, xi, vi, rhoi, pi, xj, load3(vel, j), rho[j], prs[j]);
         });
     }
-    V3 fi = V3{fx.value(), fy.value(), fz.value()} * p.mass;
-    if (p.mass_consistent) fi = fi * p.mass;
-    fi.x = cube_wall(xi.x, fi.x);
-    fi.y = cube_wall(xi.y, fi.y);
-    fi.z = cube_wall(xi.z, fi.z);
-    store3(f.force, i, fi);
+    store3(f.force, i, acc.finish(p, xi));
 }
 
 __device__ __forceinline__ bool is_static_body(const StepParams &p, const FieldPtrs &f, unsigned i) {
     if (!f.body_number) return false;
     const unsigned b = f.body_number[i];
     if (b == 0u || !f.body_dynamic || b > p.body_count) return false;
-    return f.body_dynamic[b - 1u] == 0u;
+    return f.body_dynamic[p.body_base + b - 1u] == 0u;
 }
 
-__global__ void __launch_bounds__(kBlock) k_step_forward(StepParams p, FieldPtrs f) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n || is_static_body(p, f, i)) return;
+template <class PS>
+__global__ void __launch_bounds__(kBlock) k_step_forward(PS ps, FieldPtrs f) {
+    const unsigned i = linear_slot();
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
+    if (is_static_body(p, f, i)) return;
     const V3 g = V3{p.gravity[0], p.gravity[1], p.gravity[2]};
     const V3 a = load3(f.force, i) / p.mass + g;
     if (f.acceleration) store3(f.acceleration, i, a);
@@ -337,9 +338,11 @@ __global__ void __launch_bounds__(kBlock) k_step_forward(StepParams p, FieldPtrs
     store3(f.velocity, i, vn);
 }
 
-__global__ void __launch_bounds__(kBlock) k_plane_constraints(StepParams p, FieldPtrs f) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
+template <class PS>
+__global__ void __launch_bounds__(kBlock) k_plane_constraints(PS ps, FieldPtrs f) {
+    const unsigned i = linear_slot();
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
     const unsigned mask = f.plane_constraints_particles[i];
     if (mask == 0u) return;
     V3 x = load3(f.position, i), v = load3(f.velocity, i);
@@ -354,9 +357,10 @@ __global__ void __launch_bounds__(kBlock) k_plane_constraints(StepParams p, Fiel
     store3(f.velocity, i, v);
 }
 
-__global__ void __launch_bounds__(kBlock) k_init_original_position(StepParams p, FieldPtrs f) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
+template <class PS>
+__global__ void __launch_bounds__(kBlock) k_init_original_position(PS ps, FieldPtrs f) {
+    const unsigned i = linear_slot();
+    if (!ps.active(i)) return;
     store3(f.originalpos, i, load3(f.position, i));
 }
 
@@ -404,72 +408,96 @@ inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 // Neighbour passes: a multiple of 8 blocks for the XCD-aware slot mapping.
 inline dim3 nbr_grid(unsigned n, unsigned block) { return dim3(xcd_grid((n + block - 1) / block)); }
 
+// ---- launchers shared by both parameter sources --------------------------
+template <class PS>
+hipError_t density(const PS &ps, const FieldPtrs &f, bool original, int compensated, hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    const dim3 g = nbr_grid(ps.slots(), kBlock), b(kBlock);
+    const bool k = (compensated & kKahanDensity) != 0;
+    if (original) {
+        if (k) hipLaunchKernelGGL((k_density<PS, true, true>), g, b, 0, s, ps, f);
+        else hipLaunchKernelGGL((k_density<PS, true, false>), g, b, 0, s, ps, f);
+    } else {
+        if (k) hipLaunchKernelGGL((k_density<PS, false, true>), g, b, 0, s, ps, f);
+        else hipLaunchKernelGGL((k_density<PS, false, false>), g, b, 0, s, ps, f);
+    }
+    return hipGetLastError();
+}
+
+template <class PS>
+hipError_t density_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated, hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    const dim3 g = nbr_grid(ps.slots(), kListBlock), b(kListBlock);
+    if (compensated & kKahanDensity) hipLaunchKernelGGL((k_density_listed<PS, true>), g, b, 0, s, ps, f, c);
+    else hipLaunchKernelGGL((k_density_listed<PS, false>), g, b, 0, s, ps, f, c);
+    return hipGetLastError();
+}
+
+template <class PS>
+hipError_t forces_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated, hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    const dim3 g = nbr_grid(ps.slots(), kBlock), b(kBlock);
+    if (compensated & kKahanForces) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true>), g, b, 0, s, ps, f, c);
+    else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false>), g, b, 0, s, ps, f, c);
+    return hipGetLastError();
+}
+
+template <class PS>
+hipError_t step_forward(const PS &ps, const FieldPtrs &f, hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_step_forward<PS>), grid_for(ps.slots()), dim3(kBlock), 0, s, ps, f);
+    return hipGetLastError();
+}
+
+template <class PS>
+hipError_t init_original_position(const PS &ps, const FieldPtrs &f, hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_init_original_position<PS>), grid_for(ps.slots()), dim3(kBlock), 0, s, ps, f);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    if (p.compensated & kKahanDensity)
-        hipLaunchKernelGGL((k_density<false, true>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
-    else
-        hipLaunchKernelGGL((k_density<false, false>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
-    return hipGetLastError();
+    return density(SingleScene{p}, f, false, p.compensated, s);
 }
 
 hipError_t launch_compute_density_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
                                          hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    const dim3 grid = nbr_grid(p.n, kListBlock), block(kListBlock);
-    if (p.compensated & kKahanDensity)
-        hipLaunchKernelGGL((k_density_listed<true>), grid, block, 0, s, p, f, c);
-    else
-        hipLaunchKernelGGL((k_density_listed<false>), grid, block, 0, s, p, f, c);
-    return hipGetLastError();
+    return density_listed(SingleScene{p}, f, c, p.compensated, s);
 }
 
 hipError_t launch_forces_fluids_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
                                        hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    if (p.compensated & kKahanForces)
-        hipLaunchKernelGGL((k_forces_fluids_listed<true>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f, c);
-    else
-        hipLaunchKernelGGL((k_forces_fluids_listed<false>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f, c);
-    return hipGetLastError();
+    return forces_listed(SingleScene{p}, f, c, p.compensated, s);
 }
 
 hipError_t launch_compute_original_density(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    if (p.compensated & kKahanDensity)
-        hipLaunchKernelGGL((k_density<true, true>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
-    else
-        hipLaunchKernelGGL((k_density<true, false>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
-    return hipGetLastError();
+    return density(SingleScene{p}, f, true, p.compensated, s);
 }
 
 hipError_t launch_forces_fluids(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
+    const SingleScene ps{p};
     if (p.compensated & kKahanForces)
-        hipLaunchKernelGGL((k_forces_fluids<true>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
+        hipLaunchKernelGGL((k_forces_fluids<SingleScene, true>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, ps, f);
     else
-        hipLaunchKernelGGL((k_forces_fluids<false>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, p, f);
+        hipLaunchKernelGGL((k_forces_fluids<SingleScene, false>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, ps, f);
     return hipGetLastError();
 }
 
 hipError_t launch_step_forward(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_step_forward, grid_for(p.n), dim3(kBlock), 0, s, p, f);
-    return hipGetLastError();
+    return step_forward(SingleScene{p}, f, s);
 }
 
 hipError_t launch_plane_constraints(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_plane_constraints, grid_for(p.n), dim3(kBlock), 0, s, p, f);
+    hipLaunchKernelGGL((k_plane_constraints<SingleScene>), grid_for(p.n), dim3(kBlock), 0, s, SingleScene{p}, f);
     return hipGetLastError();
 }
 
 hipError_t launch_init_original_position(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_init_original_position, grid_for(p.n), dim3(kBlock), 0, s, p, f);
-    return hipGetLastError();
+    return init_original_position(SingleScene{p}, f, s);
 }
 
 hipError_t launch_global_sums(const StepParams &p, const FieldPtrs &f, double *partials, double *out7,
@@ -477,6 +505,35 @@ hipError_t launch_global_sums(const StepParams &p, const FieldPtrs &f, double *p
     hipLaunchKernelGGL(k_sums_partial, dim3(kReduceBlocks), dim3(kBlock), 0, s, p, f, partials);
     hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(64), 0, s, partials, out7, p.n, p.mass);
     return hipGetLastError();
+}
+
+hipError_t launch_batch_density_listed(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                       const PairCache &c, int compensated, hipStream_t s) {
+    return density_listed(SceneTable{b, sel}, f, c, compensated, s);
+}
+
+hipError_t launch_batch_density(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, int compensated,
+                                hipStream_t s) {
+    return density(SceneTable{b, sel}, f, false, compensated, s);
+}
+
+hipError_t launch_batch_original_density(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                         int compensated, hipStream_t s) {
+    return density(SceneTable{b, sel}, f, true, compensated, s);
+}
+
+hipError_t launch_batch_init_original_position(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                               hipStream_t s) {
+    return init_original_position(SceneTable{b, sel}, f, s);
+}
+
+hipError_t launch_batch_forces_fluids_listed(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                             const PairCache &c, int compensated, hipStream_t s) {
+    return forces_listed(SceneTable{b, sel}, f, c, compensated, s);
+}
+
+hipError_t launch_batch_step_forward(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, hipStream_t s) {
+    return step_forward(SceneTable{b, sel}, f, s);
 }
 
 }  // namespace sphk

@@ -10,7 +10,7 @@
 // is what a sequential run of the reference produces, so gridcell, gridcount,
 // celloffset and cellparticles are deterministic and bit-exact against the
 // sequential oracle.
-#include "kernels.h"
+#include "cells.h"
 
 #pragma clang fp contract(off)
 
@@ -18,24 +18,6 @@ namespace sphk {
 namespace {
 
 constexpr unsigned kBlock = 256;
-
-__device__ __forceinline__ unsigned grid_index(float p, float lo, float hi, unsigned res) {
-    // (p - lo) / (hi - lo) * res, truncated toward zero, clamped to [0, res-1]
-    // (kernels/common.cl:384-388).  Values below 0 and NaN land in cell 0.
-    float v = (p - lo) / (hi - lo) * static_cast<float>(res);
-    unsigned u;
-    if (!(v > 0.0f)) u = 0u;
-    else if (v >= 4294967296.0f) u = 0xffffffffu;
-    else u = static_cast<unsigned>(v);
-    return u < res - 1u ? u : res - 1u;
-}
-
-__device__ __forceinline__ unsigned cell_key(const float *pos, unsigned s, const GridGeom &g) {
-    unsigned gx = grid_index(pos[3 * s], g.lo[0], g.hi[0], g.rx);
-    unsigned gy = grid_index(pos[3 * s + 1], g.lo[1], g.hi[1], g.ry);
-    unsigned gz = grid_index(pos[3 * s + 2], g.lo[2], g.hi[2], g.rz);
-    return gz * g.rx * g.ry + gy * g.rx + gx;   // (kernels/common.cl:390)
-}
 
 __global__ void k_fill_u32(unsigned *p, size_t count, unsigned value) {
     size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -56,33 +38,17 @@ __global__ void k_cell_keys(const float *__restrict__ pos, unsigned n, GridGeom 
     key[s] = cell_key(pos, s, g);
 }
 
-// Key and arrival rank of every active slot.  Slots are cell-sorted from the
-// previous step, so runs of equal keys inside a wavefront are common; each
-// run issues one atomic and hands out consecutive ranks.
+// Key and arrival rank of every active slot.
 __global__ void k_key_count(const float *__restrict__ pos, unsigned n, GridGeom g,
                             unsigned *__restrict__ key, unsigned *__restrict__ local,
                             unsigned *__restrict__ count) {
     unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = s < n;
-    unsigned k = active ? cell_key(pos, s, g) : 0xffffffffu;
-    const int lane = threadIdx.x & 63;
-    unsigned prev = __shfl_up(k, 1, 64);
-    const bool head = active && (lane == 0 || prev != k);
-    unsigned long long heads = __ballot(head);
-    unsigned long long act = __ballot(active);
+    const unsigned k = active ? cell_key(pos, s, g) : 0xffffffffu;
+    const unsigned rank = claim_ranks(k, active, count);
     if (!active) return;
-    // Run start = highest head lane <= lane; run end = next head (or last active lane).
-    unsigned long long upto = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
-    int start = 63 - __clzll(upto);
-    unsigned long long after = heads & ~((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
-    int next = after ? __ffsll(static_cast<long long>(after)) - 1 : 64;
-    int last_active = 63 - __clzll(act);
-    int end = next < last_active + 1 ? next : last_active + 1;
-    unsigned base = 0;
-    if (lane == start) base = atomicAdd(&count[k], static_cast<unsigned>(end - start));
-    base = __shfl(base, start, 64);
     key[s] = k;
-    local[s] = base + static_cast<unsigned>(lane - start);
+    local[s] = rank;
 }
 
 // ---- exclusive scan over cells -------------------------------------------

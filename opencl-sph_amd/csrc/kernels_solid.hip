@@ -110,10 +110,11 @@ __device__ __forceinline__ bool in_body(const FieldPtrs &f, unsigned i) {
     return f.body_number != nullptr && f.body_number[i] > 0u;
 }
 
-template <bool KAHAN, bool FP64>
-__global__ void __launch_bounds__(kBlock) k_rotations_strains(StepParams p, FieldPtrs f) {
-    const unsigned i = xcd_slot(blockDim.x);
-    if (i >= p.n || in_body(f, i)) return;
+template <class PS, bool KAHAN, bool FP64>
+__global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f) {
+    const unsigned i = xcd_slot(kBlock);
+    if (!ps.active(i) || in_body(f, i)) return;
+    const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
     const float *__restrict__ opos = f.originalpos;
@@ -123,7 +124,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(StepParams p, Fiel
     const unsigned key = f.gridcell[i];
 
     Acc<KAHAN> apq[9];
-    for_each_candidate(p.grid, p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
+    for_each_candidate(p.grid, p.p0, p.p0 + p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 pv = xj - xi;
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(StepParams p, Fiel
 
     const float d0i = f.density0[i];
     Acc<KAHAN> du[9];
-    for_each_candidate(p.grid, p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
+    for_each_candidate(p.grid, p.p0, p.p0 + p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 q = x0j - x0i;
@@ -189,9 +190,11 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(StepParams p, Fiel
 }
 
 // sigma = C : eps, isotropic   (kernels/solid_corotation.cl:9-26,197-208)
-__global__ void __launch_bounds__(kBlock) k_stresses_corot(StepParams p, FieldPtrs f) {
+template <class PS>
+__global__ void __launch_bounds__(kBlock) k_stresses_corot(PS ps, FieldPtrs f) {
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
     const float E = p.youngs, nu = p.poisson;
     const float lam = E * nu / (1.0f + nu) / (1.0f - 2.0f * nu);
     const float mu = E / 2.0f / (1.0f + nu);
@@ -207,10 +210,11 @@ __global__ void __launch_bounds__(kBlock) k_stresses_corot(StepParams p, FieldPt
                c[r][5] * ev[5];
 }
 
-template <bool KAHAN, bool FP64>
-__global__ void __launch_bounds__(kBlock) k_deformations_svk(StepParams p, FieldPtrs f) {
-    const unsigned i = xcd_slot(blockDim.x);
-    if (i >= p.n) return;
+template <class PS, bool KAHAN, bool FP64>
+__global__ void __launch_bounds__(kBlock) k_deformations_svk(PS ps, FieldPtrs f) {
+    const unsigned i = xcd_slot(kBlock);
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
     const float *__restrict__ opos = f.originalpos;
@@ -218,7 +222,7 @@ __global__ void __launch_bounds__(kBlock) k_deformations_svk(StepParams p, Field
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned ncombo = 1u << p.nplanes;
     Acc<KAHAN> apq[9], aqq[6];
-    for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+    for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 pv = xj - xi;
@@ -256,9 +260,11 @@ __global__ void __launch_bounds__(kBlock) k_deformations_svk(StepParams p, Field
 
 // Green strain -> SVK 2nd Piola-Kirchhoff -> Cauchy, with the double
 // literals of kernels/solid_svk.cl:1-43 kept as double arithmetic.
-__global__ void __launch_bounds__(kBlock) k_stresses_svk(StepParams p, FieldPtrs f) {
+template <class PS>
+__global__ void __launch_bounds__(kBlock) k_stresses_svk(PS ps, FieldPtrs f) {
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
     float F[9], Ft[9], ftf[9];
     for (int k = 0; k < 9; ++k) F[k] = f.deformation[9 * i + k];
     Ft[0] = F[0]; Ft[4] = F[4]; Ft[8] = F[8];
@@ -295,10 +301,11 @@ __global__ void __launch_bounds__(kBlock) k_stresses_svk(StepParams p, FieldPtrs
     s[S12] = sc * cauchy[5];
 }
 
-template <bool CORO, bool KAHAN>
-__global__ void __launch_bounds__(kBlock) k_forces_solids(StepParams p, FieldPtrs f) {
-    const unsigned i = xcd_slot(blockDim.x);
-    if (i >= p.n) return;
+template <class PS, bool CORO, bool KAHAN>
+__global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f) {
+    const unsigned i = xcd_slot(kBlock);
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
     const float *__restrict__ opos = f.originalpos;
@@ -314,7 +321,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(StepParams p, FieldPtr
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned ncombo = 1u << p.nplanes;
     Acc3<KAHAN> fe, fv, fec, fvc;
-    for_each_candidate(p.grid, p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+    for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 x0j = load3(opos, j);
         const V3 xj = load3(pos, j);
         const V3 X = x0i - x0j;
@@ -382,57 +389,108 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(StepParams p, FieldPtr
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 inline dim3 nbr_grid(unsigned n) { return dim3(xcd_grid((n + kBlock - 1) / kBlock)); }
 
+template <class PS>
+hipError_t rotations_and_strains(const PS &ps, const FieldPtrs &f, int compensated, int polar_fp64,
+                                 hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    const dim3 g = nbr_grid(ps.slots()), b(kBlock);
+    if (compensated & kKahanForces) {
+        if (polar_fp64) hipLaunchKernelGGL((k_rotations_strains<PS, true, true>), g, b, 0, s, ps, f);
+        else hipLaunchKernelGGL((k_rotations_strains<PS, true, false>), g, b, 0, s, ps, f);
+    } else {
+        if (polar_fp64) hipLaunchKernelGGL((k_rotations_strains<PS, false, true>), g, b, 0, s, ps, f);
+        else hipLaunchKernelGGL((k_rotations_strains<PS, false, false>), g, b, 0, s, ps, f);
+    }
+    return hipGetLastError();
+}
+
+template <class PS>
+hipError_t stresses_corotational(const PS &ps, const FieldPtrs &f, hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_stresses_corot<PS>), grid_for(ps.slots()), dim3(kBlock), 0, s, ps, f);
+    return hipGetLastError();
+}
+
+template <class PS>
+hipError_t deformations_svk(const PS &ps, const FieldPtrs &f, int compensated, int polar_fp64, hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    const dim3 g = nbr_grid(ps.slots()), b(kBlock);
+    if (compensated & kKahanForces) {
+        if (polar_fp64) hipLaunchKernelGGL((k_deformations_svk<PS, true, true>), g, b, 0, s, ps, f);
+        else hipLaunchKernelGGL((k_deformations_svk<PS, true, false>), g, b, 0, s, ps, f);
+    } else {
+        if (polar_fp64) hipLaunchKernelGGL((k_deformations_svk<PS, false, true>), g, b, 0, s, ps, f);
+        else hipLaunchKernelGGL((k_deformations_svk<PS, false, false>), g, b, 0, s, ps, f);
+    }
+    return hipGetLastError();
+}
+
+template <class PS>
+hipError_t stresses_svk(const PS &ps, const FieldPtrs &f, hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_stresses_svk<PS>), grid_for(ps.slots()), dim3(kBlock), 0, s, ps, f);
+    return hipGetLastError();
+}
+
+template <class PS>
+hipError_t forces_solids(const PS &ps, const FieldPtrs &f, bool corotational, int compensated, hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    const dim3 g = nbr_grid(ps.slots()), b(kBlock);
+    const bool k = (compensated & kKahanForces) != 0;
+    if (corotational) {
+        if (k) hipLaunchKernelGGL((k_forces_solids<PS, true, true>), g, b, 0, s, ps, f);
+        else hipLaunchKernelGGL((k_forces_solids<PS, true, false>), g, b, 0, s, ps, f);
+    } else {
+        if (k) hipLaunchKernelGGL((k_forces_solids<PS, false, true>), g, b, 0, s, ps, f);
+        else hipLaunchKernelGGL((k_forces_solids<PS, false, false>), g, b, 0, s, ps, f);
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_rotations_and_strains(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    dim3 g = nbr_grid(p.n), b(kBlock);
-    if (p.compensated & kKahanForces) {
-        if (p.polar_fp64) hipLaunchKernelGGL((k_rotations_strains<true, true>), g, b, 0, s, p, f);
-        else hipLaunchKernelGGL((k_rotations_strains<true, false>), g, b, 0, s, p, f);
-    } else {
-        if (p.polar_fp64) hipLaunchKernelGGL((k_rotations_strains<false, true>), g, b, 0, s, p, f);
-        else hipLaunchKernelGGL((k_rotations_strains<false, false>), g, b, 0, s, p, f);
-    }
-    return hipGetLastError();
+    return rotations_and_strains(SingleScene{p}, f, p.compensated, p.polar_fp64, s);
 }
 
 hipError_t launch_stresses_corotational(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stresses_corot, grid_for(p.n), dim3(kBlock), 0, s, p, f);
-    return hipGetLastError();
+    return stresses_corotational(SingleScene{p}, f, s);
 }
 
 hipError_t launch_deformations_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    dim3 g = nbr_grid(p.n), b(kBlock);
-    if (p.compensated & kKahanForces) {
-        if (p.polar_fp64) hipLaunchKernelGGL((k_deformations_svk<true, true>), g, b, 0, s, p, f);
-        else hipLaunchKernelGGL((k_deformations_svk<true, false>), g, b, 0, s, p, f);
-    } else {
-        if (p.polar_fp64) hipLaunchKernelGGL((k_deformations_svk<false, true>), g, b, 0, s, p, f);
-        else hipLaunchKernelGGL((k_deformations_svk<false, false>), g, b, 0, s, p, f);
-    }
-    return hipGetLastError();
+    return deformations_svk(SingleScene{p}, f, p.compensated, p.polar_fp64, s);
 }
 
 hipError_t launch_stresses_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stresses_svk, grid_for(p.n), dim3(kBlock), 0, s, p, f);
-    return hipGetLastError();
+    return stresses_svk(SingleScene{p}, f, s);
 }
 
 hipError_t launch_forces_solids(const StepParams &p, const FieldPtrs &f, bool corotational, hipStream_t s) {
-    if (p.n == 0) return hipSuccess;
-    dim3 g = nbr_grid(p.n), b(kBlock);
-    if (corotational) {
-        if (p.compensated & kKahanForces) hipLaunchKernelGGL((k_forces_solids<true, true>), g, b, 0, s, p, f);
-        else hipLaunchKernelGGL((k_forces_solids<true, false>), g, b, 0, s, p, f);
-    } else {
-        if (p.compensated & kKahanForces) hipLaunchKernelGGL((k_forces_solids<false, true>), g, b, 0, s, p, f);
-        else hipLaunchKernelGGL((k_forces_solids<false, false>), g, b, 0, s, p, f);
-    }
-    return hipGetLastError();
+    return forces_solids(SingleScene{p}, f, corotational, p.compensated, s);
+}
+
+hipError_t launch_batch_rotations_and_strains(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                              int compensated, int polar_fp64, hipStream_t s) {
+    return rotations_and_strains(SceneTable{b, sel}, f, compensated, polar_fp64, s);
+}
+
+hipError_t launch_batch_stresses_corotational(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                              hipStream_t s) {
+    return stresses_corotational(SceneTable{b, sel}, f, s);
+}
+
+hipError_t launch_batch_deformations_svk(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                         int compensated, int polar_fp64, hipStream_t s) {
+    return deformations_svk(SceneTable{b, sel}, f, compensated, polar_fp64, s);
+}
+
+hipError_t launch_batch_stresses_svk(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, hipStream_t s) {
+    return stresses_svk(SceneTable{b, sel}, f, s);
+}
+
+hipError_t launch_batch_forces_solids(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                      bool corotational, int compensated, hipStream_t s) {
+    return forces_solids(SceneTable{b, sel}, f, corotational, compensated, s);
 }
 
 }  // namespace sphk

@@ -11,6 +11,7 @@
 
 #include "device_math.h"
 #include "kernels.h"
+#include "params.h"
 
 namespace sphk {
 
@@ -32,13 +33,14 @@ __device__ __forceinline__ unsigned xcd_slot(unsigned block_threads) {
 }
 
 template <typename Body>
-__device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned n, unsigned key,
+__device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned lo, unsigned hi, unsigned key,
                                                    const unsigned *__restrict__ celloffset,
                                                    const unsigned *__restrict__ gridcount,
                                                    Body &&body) {
-    if (key >= g.ncells) return;
-    const unsigned zo = key / g.layer;
-    const unsigned rem = key - g.layer * zo;
+    if (key < g.c0 || key - g.c0 >= g.ncells) return;
+    const unsigned local = key - g.c0;
+    const unsigned zo = local / g.layer;
+    const unsigned rem = local - g.layer * zo;
     const unsigned yo = rem / g.rx;
     const unsigned xo = rem - g.rx * yo;
     const unsigned x0 = xo > 0u ? xo - 1u : 0u;
@@ -51,11 +53,11 @@ __device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned n
         for (int dy = -1; dy <= 1; ++dy) {
             const int yn = static_cast<int>(yo) + dy;
             if (yn < 0 || yn >= static_cast<int>(g.ry)) continue;
-            const unsigned row = static_cast<unsigned>(zn) * g.layer + static_cast<unsigned>(yn) * g.rx;
-            // Ranges are clamped to the active slots so that grid arrays
+            const unsigned row = g.c0 + static_cast<unsigned>(zn) * g.layer + static_cast<unsigned>(yn) * g.rx;
+            // Ranges are clamped to the scene's slots so that grid arrays
             // uploaded from the host can never index past the particle data.
-            const unsigned b = min(celloffset[row + x0], n);
-            const unsigned e = min(celloffset[row + x1] + gridcount[row + x1], n);
+            const unsigned b = min(max(celloffset[row + x0], lo), hi);
+            const unsigned e = min(max(celloffset[row + x1] + gridcount[row + x1], lo), hi);
             for (unsigned j = b; j < e; ++j) body(j);
         }
     }

@@ -27,29 +27,17 @@
 
 #include "host_common.h"
 #include "kernels.h"
+#include "runtime_internal.h"
 
 using sphhip::FieldKind;
+using sphhip::KernelSet;
+using sphhip::fail;
 
 namespace {
 
 thread_local std::string t_error;
 
-int fail(const char *fmt, ...) {
-    char buf[1024];
-    va_list args;
-    va_start(args, fmt);
-    std::vsnprintf(buf, sizeof buf, fmt, args);
-    va_end(args);
-    t_error = buf;
-    note(2, "sph_hip: %s\n", buf);
-    return -1;
-}
-
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t err_ = (expr);                                                         \
-        if (err_ != hipSuccess) return fail("%s: %s", #expr, hipGetErrorString(err_));    \
-    } while (0)
+#define HIP_TRY(expr) SPH_HIP_TRY(expr)
 
 bool g_ready = false;
 int g_device = 0;
@@ -77,11 +65,6 @@ struct KernelStat {
 struct PendingTiming {
     std::string name;
     hipEvent_t start, stop;
-};
-
-struct KernelSet {
-    bool common = false, fluids = false, solids = false, corot = false, svk = false,
-         constraints = false, rigidbody = false;
 };
 
 }  // namespace
@@ -120,118 +103,9 @@ struct psdata_hip {
 
 namespace {
 
-// ---- small-field values (as the device holds them) -------------------------
-const char *shadow_ptr(const psdata_hip *h, const DevField *f) {
-    return h->shadow.data() + h->host.data_offsets[f->index];
-}
-bool read_float(const psdata_hip *h, DevField *f, unsigned k, float *out) {
-    if (!f) return false;
-    unsigned es = h->host.entry_sizes[f->index];
-    if (static_cast<size_t>(k + 1) * es > f->bytes) return false;
-    const char *p = shadow_ptr(h, f) + static_cast<size_t>(k) * es;
-    if (es == 8) {
-        double d;
-        std::memcpy(&d, p, 8);
-        *out = static_cast<float>(d);
-    } else {
-        std::memcpy(out, p, 4);
-    }
-    return true;
-}
-bool read_u32(const psdata_hip *h, DevField *f, unsigned k, unsigned *out) {
-    if (!f || static_cast<size_t>(k + 1) * 4 > f->bytes) return false;
-    std::memcpy(out, shadow_ptr(h, f) + static_cast<size_t>(k) * 4, 4);
-    return true;
-}
-float scalar_or(psdata_hip *h, const char *name, float dflt) {
-    float v;
-    return read_float(h, h->field(name), 0, &v) ? v : dflt;
-}
-unsigned last_dim(const psdata_hip *h, const DevField *f) {
-    unsigned nd = h->host.num_dimensions[f->index];
-    if (nd == 0) return 0;
-    return h->host.dimensions[h->host.dimensions_offsets[f->index] + nd - 1];
-}
-
 int build_params(psdata_hip *h, sphk::StepParams *pp) {
-    sphk::StepParams p{};
-    p.n = h->n;
-    if (h->has_grid) {
-        DevField *gr = h->field("gridres");
-        unsigned r[3];
-        for (unsigned a = 0; a < 3; ++a)
-            if (!read_u32(h, gr, a, &r[a])) return fail("gridres is missing");
-        p.grid.rx = r[0];
-        p.grid.ry = r[1];
-        p.grid.rz = r[2];
-        p.grid.layer = r[0] * r[1];
-        p.grid.ncells = r[0] * r[1] * r[2];
-        DevField *gb = h->field("gridbounds");
-        for (unsigned a = 0; a < 3; ++a) {
-            if (!read_float(h, gb, 2 * a, &p.grid.lo[a]) || !read_float(h, gb, 2 * a + 1, &p.grid.hi[a]))
-                return fail("gridbounds needs 6 values");
-        }
-    }
-    const float hh = scalar_or(h, "smoothingradius", 0.0f);
-    p.h = hh;
-    p.h2 = hh * hh;
-    // 315/(64 pi h^9), -45/(pi h^6), 45/(pi h^6): double, rounded once.
-    const double hd = static_cast<double>(hh);
-    const double h6 = hd * hd * hd * hd * hd * hd, h9 = h6 * hd * hd * hd;
-    const double pi = 3.14159265358979323846;
-    p.poly6 = static_cast<float>(315.0 / (64.0 * pi * h9));
-    p.spiky = static_cast<float>(-45.0 / (pi * h6));
-    p.lap = static_cast<float>(45.0 / (pi * h6));
-    // Smallest float s with sqrtf(s) > h (sqrtf is correctly rounded on the
-    // host and on the device, and monotonic).
-    float s2 = hh * hh;
-    if (hh > 0.0f && std::isfinite(hh)) {
-        while (s2 > 0.0f && std::sqrt(s2) > hh) s2 = std::nextafter(s2, 0.0f);
-        while (!(std::sqrt(s2) > hh)) s2 = std::nextafter(s2, INFINITY);
-    } else {
-        s2 = INFINITY;
-    }
-    p.r2_out = s2;
-    p.mass = scalar_or(h, "mass", 0.0f);
-    p.dt = scalar_or(h, "timestep", 0.0f);
-    p.restdens = scalar_or(h, "restdens", 0.0f);
-    p.stiffness = scalar_or(h, "stiffness", 0.0f);
-    p.viscosity = scalar_or(h, "viscosity", 0.0f);
-    p.youngs = scalar_or(h, "youngs_modulus", 0.0f);
-    p.poisson = scalar_or(h, "poisson", 0.0f);
-    DevField *g = h->field("gravity");
-    for (unsigned a = 0; a < 3; ++a)
-        if (!read_float(h, g, a, &p.gravity[a])) p.gravity[a] = 0.0f;
-
-    if (DevField *sp = h->field("symmetry_planes")) {
-        unsigned np = last_dim(h, sp);
-        if (np > static_cast<unsigned>(sphk::kMaxSymmetryPlanes))
-            return fail("at most %d symmetry planes are supported (conf has %u)", sphk::kMaxSymmetryPlanes, np);
-        if (static_cast<size_t>(np) * 6 * 4 > sp->bytes) return fail("symmetry_planes is smaller than 6 x planes");
-        p.nplanes = np;
-        for (unsigned s = 0; s < np; ++s) {
-            for (unsigned k = 0; k < 6; ++k) read_float(h, sp, 6 * s + k, &p.planes[s][k]);
-            if (p.planes[s][3] != 0.0f || p.planes[s][4] != 0.0f || p.planes[s][5] != 0.0f)
-                p.active_planes |= 1u << s;
-        }
-    }
-    if (DevField *pc = h->field("plane_constraints")) {
-        unsigned nc = last_dim(h, pc);
-        if (nc > static_cast<unsigned>(sphk::kMaxPlaneConstraints)) nc = sphk::kMaxPlaneConstraints;
-        if (static_cast<size_t>(nc) * 6 * 4 > pc->bytes) return fail("plane_constraints is smaller than 6 x planes");
-        p.nconstraints = nc;
-        for (unsigned c = 0; c < nc; ++c)
-            for (unsigned k = 0; k < 6; ++k) read_float(h, pc, 6 * c + k, &p.constraints[c][k]);
-    }
-    if (DevField *bd = h->field("body_dynamic")) p.body_count = static_cast<unsigned>(bd->bytes / 4);
-    unsigned flag = 0;
-    p.eos_tait = read_u32(h, h->field("eos_tait"), 0, &flag) && flag != 0u;
-    flag = 0;
-    p.mass_consistent = read_u32(h, h->field("mass_consistent"), 0, &flag) && flag != 0u;
-    p.compensated = h->compensated;
-    p.polar_fp64 = h->polar_fp64;
-    *pp = p;
-    return 0;
+    return sphhip::params_from_blob(h->host, h->shadow.data(), h->kset, h->n, h->has_grid, h->compensated,
+                                    h->polar_fp64, pp);
 }
 
 template <typename T>
@@ -623,29 +497,6 @@ bool hot_field(const std::string &name, const KernelSet &k) {
     return false;
 }
 
-KernelSet parse_file_list(const char *list) {
-    KernelSet k;
-    if (!list) return k;
-    std::string s(list);
-    size_t i = 0;
-    while (i < s.size()) {
-        while (i < s.size() && (s[i] == ',' || s[i] == ' ' || s[i] == '\n' || s[i] == '\r' || s[i] == '\t')) ++i;
-        size_t j = i;
-        while (j < s.size() && !(s[j] == ',' || s[j] == ' ' || s[j] == '\n' || s[j] == '\r' || s[j] == '\t')) ++j;
-        std::string w = s.substr(i, j - i);
-        if (w == "common") k.common = true;
-        else if (w == "fluids") k.fluids = true;
-        else if (w == "solids") k.solids = true;
-        else if (w == "solid_corotation") k.corot = true;
-        else if (w == "solid_svk") k.svk = true;
-        else if (w == "constraints") k.constraints = true;
-        else if (w == "rigidbody") k.rigidbody = true;
-        else if (!w.empty()) note(1, "sph_hip: ignoring unknown kernel file '%s'\n", w.c_str());
-        i = j;
-    }
-    return k;
-}
-
 void release(psdata_hip *h) {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
@@ -736,6 +587,181 @@ bool same_psdata(const psdata &a, const psdata &b) { return a.data == b.data && 
 }  // namespace
 
 // ============================================================================
+// Helpers shared with the scene batch
+// ============================================================================
+namespace sphhip {
+
+int fail(const char *fmt, ...) {
+    char buf[1024];
+    va_list args;
+    va_start(args, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, args);
+    va_end(args);
+    t_error = buf;
+    note(2, "sph_hip: %s\n", buf);
+    return -1;
+}
+
+KernelSet parse_file_list(const char *list) {
+    KernelSet k;
+    if (!list) return k;
+    std::string s(list);
+    size_t i = 0;
+    while (i < s.size()) {
+        while (i < s.size() && (s[i] == ',' || s[i] == ' ' || s[i] == '\n' || s[i] == '\r' || s[i] == '\t')) ++i;
+        size_t j = i;
+        while (j < s.size() && !(s[j] == ',' || s[j] == ' ' || s[j] == '\n' || s[j] == '\r' || s[j] == '\t')) ++j;
+        std::string w = s.substr(i, j - i);
+        if (w == "common") k.common = true;
+        else if (w == "fluids") k.fluids = true;
+        else if (w == "solids") k.solids = true;
+        else if (w == "solid_corotation") k.corot = true;
+        else if (w == "solid_svk") k.svk = true;
+        else if (w == "constraints") k.constraints = true;
+        else if (w == "rigidbody") k.rigidbody = true;
+        else if (!w.empty()) note(1, "sph_hip: ignoring unknown kernel file '%s'\n", w.c_str());
+        i = j;
+    }
+    return k;
+}
+
+int current_device(int *device) {
+    if (!g_ready && init_hip()) return -1;
+    *device = g_device;
+    return 0;
+}
+
+namespace {
+
+// Field values as the blob holds them.
+struct BlobReader {
+    const psdata &meta;
+    const char *blob;
+    int index(const char *name) const { return get_field_psdata(meta, name); }
+    bool read_float(int f, unsigned k, float *out) const {
+        if (f < 0) return false;
+        const unsigned es = meta.entry_sizes[f];
+        if (static_cast<size_t>(k + 1) * es > meta.data_sizes[f]) return false;
+        const char *p = blob + meta.data_offsets[f] + static_cast<size_t>(k) * es;
+        if (es == 8) {
+            double d;
+            std::memcpy(&d, p, 8);
+            *out = static_cast<float>(d);
+        } else {
+            std::memcpy(out, p, 4);
+        }
+        return true;
+    }
+    bool read_u32(int f, unsigned k, unsigned *out) const {
+        if (f < 0 || static_cast<size_t>(k + 1) * 4 > meta.data_sizes[f]) return false;
+        std::memcpy(out, blob + meta.data_offsets[f] + static_cast<size_t>(k) * 4, 4);
+        return true;
+    }
+    float scalar_or(const char *name, float dflt) const {
+        float v;
+        return read_float(index(name), 0, &v) ? v : dflt;
+    }
+    unsigned last_dim(int f) const {
+        const unsigned nd = meta.num_dimensions[f];
+        if (nd == 0) return 0;
+        return meta.dimensions[meta.dimensions_offsets[f] + nd - 1];
+    }
+};
+
+}  // namespace
+
+int params_from_blob(const psdata &meta, const char *blob, const KernelSet &ks, unsigned n, bool has_grid,
+                     int compensated, int polar_fp64, sphk::StepParams *pp) {
+    const BlobReader r{meta, blob};
+    sphk::StepParams p{};
+    p.n = n;
+    if (has_grid) {
+        const int gr = r.index("gridres");
+        unsigned res[3];
+        for (unsigned a = 0; a < 3; ++a)
+            if (!r.read_u32(gr, a, &res[a])) return fail("gridres is missing");
+        p.grid.rx = res[0];
+        p.grid.ry = res[1];
+        p.grid.rz = res[2];
+        p.grid.layer = res[0] * res[1];
+        p.grid.ncells = res[0] * res[1] * res[2];
+        const int gb = r.index("gridbounds");
+        for (unsigned a = 0; a < 3; ++a) {
+            if (!r.read_float(gb, 2 * a, &p.grid.lo[a]) || !r.read_float(gb, 2 * a + 1, &p.grid.hi[a]))
+                return fail("gridbounds needs 6 values");
+        }
+    }
+    const float hh = r.scalar_or("smoothingradius", 0.0f);
+    p.h = hh;
+    p.h2 = hh * hh;
+    // 315/(64 pi h^9), -45/(pi h^6), 45/(pi h^6): double, rounded once.
+    const double hd = static_cast<double>(hh);
+    const double h6 = hd * hd * hd * hd * hd * hd, h9 = h6 * hd * hd * hd;
+    const double pi = 3.14159265358979323846;
+    p.poly6 = static_cast<float>(315.0 / (64.0 * pi * h9));
+    p.spiky = static_cast<float>(-45.0 / (pi * h6));
+    p.lap = static_cast<float>(45.0 / (pi * h6));
+    // Smallest float s with sqrtf(s) > h (sqrtf is correctly rounded on the
+    // host and on the device, and monotonic).
+    float s2 = hh * hh;
+    if (hh > 0.0f && std::isfinite(hh)) {
+        while (s2 > 0.0f && std::sqrt(s2) > hh) s2 = std::nextafter(s2, 0.0f);
+        while (!(std::sqrt(s2) > hh)) s2 = std::nextafter(s2, INFINITY);
+    } else {
+        s2 = INFINITY;
+    }
+    p.r2_out = s2;
+    p.mass = r.scalar_or("mass", 0.0f);
+    p.dt = r.scalar_or("timestep", 0.0f);
+    p.restdens = r.scalar_or("restdens", 0.0f);
+    p.stiffness = r.scalar_or("stiffness", 0.0f);
+    p.viscosity = r.scalar_or("viscosity", 0.0f);
+    p.youngs = r.scalar_or("youngs_modulus", 0.0f);
+    p.poisson = r.scalar_or("poisson", 0.0f);
+    const int g = r.index("gravity");
+    for (unsigned a = 0; a < 3; ++a)
+        if (!r.read_float(g, a, &p.gravity[a])) p.gravity[a] = 0.0f;
+
+    const int sp = r.index("symmetry_planes");
+    if (sp >= 0) {
+        const unsigned np = r.last_dim(sp);
+        if (np > static_cast<unsigned>(sphk::kMaxSymmetryPlanes))
+            return fail("at most %d symmetry planes are supported (conf has %u)", sphk::kMaxSymmetryPlanes, np);
+        if (static_cast<size_t>(np) * 6 * meta.entry_sizes[sp] > meta.data_sizes[sp])
+            return fail("symmetry_planes is smaller than 6 x planes");
+        p.nplanes = np;
+        for (unsigned s = 0; s < np; ++s) {
+            for (unsigned k = 0; k < 6; ++k) r.read_float(sp, 6 * s + k, &p.planes[s][k]);
+            if (p.planes[s][3] != 0.0f || p.planes[s][4] != 0.0f || p.planes[s][5] != 0.0f)
+                p.active_planes |= 1u << s;
+        }
+    }
+    const int pc = r.index("plane_constraints");
+    if (pc >= 0) {
+        unsigned nc = r.last_dim(pc);
+        if (nc > static_cast<unsigned>(sphk::kMaxPlaneConstraints)) nc = sphk::kMaxPlaneConstraints;
+        if (static_cast<size_t>(nc) * 6 * meta.entry_sizes[pc] > meta.data_sizes[pc])
+            return fail("plane_constraints is smaller than 6 x planes");
+        p.nconstraints = nc;
+        for (unsigned c = 0; c < nc; ++c)
+            for (unsigned k = 0; k < 6; ++k) r.read_float(pc, 6 * c + k, &p.constraints[c][k]);
+    }
+    const int bd = r.index("body_dynamic");
+    if (bd >= 0) p.body_count = meta.data_sizes[bd] / 4;
+    unsigned flag = 0;
+    p.eos_tait = r.read_u32(r.index("eos_tait"), 0, &flag) && flag != 0u;
+    flag = 0;
+    p.mass_consistent = r.read_u32(r.index("mass_consistent"), 0, &flag) && flag != 0u;
+    p.system = ks.corot ? sphk::kCorotational : ks.svk ? sphk::kSvk : sphk::kFluid;
+    p.compensated = compensated;
+    p.polar_fp64 = polar_fp64;
+    *pp = p;
+    return 0;
+}
+
+}  // namespace sphhip
+
+// ============================================================================
 // C ABI
 // ============================================================================
 extern "C" const char *sph_hip_last_error(void) { return t_error.c_str(); }
@@ -770,7 +796,7 @@ extern "C" int create_psdata_hip(psdata *data, const char *file_list, psdata_hip
     HIP_TRY(hipSetDevice(g_device));
     psdata_hip *h = new psdata_hip();
     h->host = *data;
-    h->kset = parse_file_list(file_list);
+    h->kset = sphhip::parse_file_list(file_list);
     if (!h->kset.common) {
         release(h);
         return fail("the kernel file list must include 'common'");
@@ -871,6 +897,7 @@ extern "C" int create_psdata_hip(psdata *data, const char *file_list, psdata_hip
         return fail("out of device memory (scratch)");
     }
     h->staging_bytes = max_particle_bytes;
+    h->pairs.stride = h->pnum;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         release(h);
         return fail("hipStreamCreate failed");

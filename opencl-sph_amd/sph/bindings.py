@@ -226,3 +226,75 @@ class PsdataHip:
         out = (c_double * 7)()
         check(_global_sums(self.handle, out))
         return list(out[0:3]), out[3], list(out[4:7])
+
+
+# ---- scene batch ----------------------------------------------------------
+_batch_bytes = _fn("sph_batch_device_bytes", [c_uint, c_uint, c_uint], c_size_t)
+_batch_create = _fn("sph_batch_create", [c_uint, c_uint, c_uint, c_int, c_int, POINTER(c_void_p)], c_int)
+_batch_free = _fn("sph_batch_free", [c_void_p], c_int)
+_batch_add = _fn("sph_batch_add", [c_void_p, POINTER(psdata), c_char_p, c_uint, POINTER(c_int)], c_int)
+_batch_remove = _fn("sph_batch_remove", [c_void_p, c_int], c_int)
+_batch_step = _fn("sph_batch_step", [c_void_p, c_uint], c_int)
+_batch_graph = _fn("sph_batch_set_graph_steps", [c_void_p, c_uint], c_int)
+_batch_sync = _fn("sph_batch_synchronize", [c_void_p], c_int)
+_batch_left = _fn("sph_batch_steps_left", [c_void_p, c_int, POINTER(c_uint)], c_int)
+_batch_read = _fn("sph_batch_read", [c_void_p, c_int, psdata, c_size_t, POINTER(c_char_p)], c_int)
+_batch_info = _fn("sph_batch_info", [c_void_p, POINTER(c_ulonglong)], c_int)
+
+
+def batch_device_bytes(slots, cells, scenes):
+    return int(_batch_bytes(slots, cells, scenes))
+
+
+class SphBatch:
+    """Device pool of resident scenes stepped together (include/sph_hip.h)."""
+
+    def __init__(self, slots, cells, scenes, compensated_sums=0, polar_fp64=True):
+        flags = 3 if compensated_sums is True else int(compensated_sums)
+        handle = c_void_p()
+        check(_batch_create(slots, cells, scenes, flags, int(bool(polar_fp64)), ctypes.byref(handle)))
+        self.handle = handle
+
+    def free(self):
+        if self.handle:
+            check(_batch_free(self.handle))
+            self.handle = c_void_p()
+
+    def add(self, data, file_list, steps):
+        """Scene handle, or None when the pool has no room now."""
+        if isinstance(file_list, str):
+            file_list = file_list.encode()
+        scene = c_int(-1)
+        rc = _batch_add(self.handle, ctypes.byref(data), file_list, int(steps), ctypes.byref(scene))
+        if rc == 1:
+            return None
+        check(rc)
+        return scene.value
+
+    def remove(self, scene):
+        check(_batch_remove(self.handle, int(scene)))
+
+    def step(self, num_steps=1):
+        check(_batch_step(self.handle, int(num_steps)))
+
+    def set_graph_steps(self, steps_per_graph):
+        check(_batch_graph(self.handle, int(steps_per_graph)))
+
+    def synchronize(self):
+        check(_batch_sync(self.handle))
+
+    def steps_left(self, scene):
+        out = c_uint()
+        check(_batch_left(self.handle, int(scene), ctypes.byref(out)))
+        return out.value
+
+    def read(self, scene, data, names):
+        arr = _names_array(names)
+        check(_batch_read(self.handle, int(scene), data, len(names), arr))
+
+    def info(self):
+        out = (c_ulonglong * 8)()
+        check(_batch_info(self.handle, out))
+        keys = ("scenes", "running", "used_slots", "used_cells", "slot_capacity", "cell_capacity",
+                "graph_captures", "graph_launches")
+        return dict(zip(keys, (int(v) for v in out)))
