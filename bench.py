@@ -1,12 +1,25 @@
 """Benchmark: particle-steps/s of the SPH step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dam_break_4m]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload served_stream]
 
-A step is one full simulation step (neighbour grid, density, forces,
-integrator) over the whole scene, with the state resident in HBM.  Prints one
-JSON line with the throughput, the roofline of the dominant kernel (HIP-event
-timing of each kernel over a second pass of the same K steps) and a CPU
-baseline (the FP32 oracle on the host cores, bounded sample).
+Default workload: the served scene stream -- seeded requests (50% fluid,
+30% corotational, 20% SVK blocks, 1k-262k particles log-uniform, K = 200
+steps each) run through the device batch (continuous batching, hipGraph
+step replay, HBM-budget admission) and, for comparison, one scene at a
+time through the single-scene path.  `value` is the batch's throughput with
+every request queued at the start (saturated); the same line reports the
+single-scene path on a prefix of the stream, the batch on that prefix, and
+p50/p99 request latency with Poisson arrivals at --rate per second.
+Scenes, results and precision settings are identical between the two paths
+(bitwise, as the GPU tests check), so the comparison is at fixed error.
+
+Single-scene workloads (--workload dam_break_4m etc.): a step is one full
+simulation step (neighbour grid, density, forces, integrator) over the whole
+scene with its state resident in HBM.
+
+Each line also carries the roofline of the dominant kernel (HIP-event timing
+over a profiled pass) and a CPU baseline (the FP32 oracle on the host cores,
+bounded sample).
 """
 import argparse
 import json
@@ -18,7 +31,8 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "opencl-sph_amd"))
 
 METRIC = "particle-steps/s batched vs plain at fixed FP64-oracle error; p50/p99 latency"
-WORKLOADS = {"dam_break_16k": 0, "cantilever_16k": 1, "svk_cube_262k": 2, "dam_break_4m": 3}
+WORKLOADS = {"dam_break_16k": 0, "cantilever_16k": 1, "svk_cube_262k": 2, "dam_break_4m": 3, "served_stream": 4}
+SUMS = {"plain": 0, "kahan-density": 1, "kahan-all": 3}
 
 
 def log(msg):
@@ -85,65 +99,59 @@ def copy_probe(mib):
     log(f"copy probe: 3 x {mib} MiB device-to-device")
 
 
-def cpu_baseline(scene, budget_s):
-    """The FP32 oracle on the host cores, full scene, as many whole steps as
-    fit the time budget (at least one after a warm-up step)."""
+def cpu_baseline(scenes_, budget_s):
+    """The FP32 oracle on the host cores over the given scenes (reference
+    configuration set up for solids, one warm-up step each), as many whole
+    steps as fit an equal share of the time budget."""
     import ctypes
 
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
-    fields = {k: v for k, v in scene.fields.items() if k in oracle.OracleState.PER_PARTICLE}
-    st = oracle.OracleState("f32", scene.params, **fields)
     lib, _, _ = oracle.load("f32")
     lib.orc_threads.restype = ctypes.c_uint
     threads = int(lib.orc_threads())
-    st.step(scene.system, 1)                        # warm-up
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        st.step(scene.system, 1)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or steps >= scene.steps:
-            break
-    return {"value": scene.n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} step(s) of the full {scene.n}-particle {scene.name} scene, FP32 oracle "
-                      f"(C restatement of the reference kernels, OpenMP), after 1 warm-up step, "
-                      f"{el:.1f} s wall"}
+    work = elapsed = 0.0
+    parts = []
+    for scene in scenes_:
+        fields = {k: v for k, v in scene.fields.items() if k in oracle.OracleState.PER_PARTICLE}
+        for k in ("body_number", "body_dynamic"):
+            if k in scene.fields:
+                fields[k] = scene.fields[k]
+        params = dict(scene.params)
+        params["polar_fp64"] = 1
+        st = oracle.OracleState("f32", params, **fields)
+        if scene.system in ("corotational", "svk"):
+            st.setup_solid()
+        st.step(scene.system, 1)                    # warm-up
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            st.step(scene.system, 1)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s / len(scenes_) or steps >= scene.steps:
+                break
+        work += scene.n * steps
+        elapsed += el
+        parts.append(f"{steps} step(s) of {scene.name} ({scene.system}, {scene.n} particles)")
+    return {"value": work / elapsed, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "; ".join(parts) + f"; FP32 oracle (C restatement of the reference kernels, OpenMP), "
+                      f"after 1 warm-up step each, {elapsed:.1f} s wall"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="dam_break_4m", choices=sorted(WORKLOADS))
-    ap.add_argument("--sums", default="kahan-density", choices=["plain", "kahan-density", "kahan-all"],
-                    help="neighbour-sum precision (default: Kahan-compensated density sums)")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work (0: skip)")
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="first run a device copy of this many MiB (counter calibration under rocprofv3)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: the engine runs one scene set per device and this build has no "
-              f"multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
-        sys.exit(2)
-
+def run_single(args):
     import numpy as np
 
     import sph
-    from sph import bindings, roofline, scenes
+    from sph import roofline, scenes
 
-    bindings.set_log_level(3)          # library notes would share stdout with the JSON line
-    sph.init_hip()
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
     scene = scenes.baseline_config(WORKLOADS[args.workload])
     log(f"workload {scene.name}: {scene.n} particles, system {scene.system}")
     system = scenes.build(scene)
     handle = system.psdata_opencl
-    handle.set_precision(compensated_sums={"plain": 0, "kahan-density": 1, "kahan-all": 3}[args.sums],
-                         polar_fp64=True)
+    handle.set_precision(compensated_sums=SUMS[args.sums], polar_fp64=True)
 
     system.run(args.warmup)
     handle.synchronize()
@@ -195,7 +203,7 @@ def main():
     cpu = None
     if args.cpu_budget > 0:
         log("CPU baseline (FP32 oracle) ...")
-        cpu = cpu_baseline(scene, args.cpu_budget)
+        cpu = cpu_baseline([scene], args.cpu_budget)
         log(f"CPU baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} threads")
 
     line = {
@@ -209,6 +217,134 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
     }
     line.update(extra)
+    return line
+
+
+def _brief(rep):
+    keep = ("requests", "particle_steps", "wall_s", "particle_steps_per_s", "latency_p50_s", "latency_p99_s",
+            "latency_mean_s", "latency_max_s", "peak_resident_slots", "peak_resident_scenes", "batch_steps",
+            "graph_captures", "graph_launches", "pool_bytes")
+    return {k: (round(v, 5) if isinstance(v, float) else v) for k, v in rep.items() if k in keep}
+
+
+def run_served(args):
+    from sph import roofline, serve
+
+    steps = args.steps if args.steps_set else 200
+    sums = SUMS[args.sums]
+    reqs = serve.make_requests(count=args.requests, rate=args.rate, seed=args.seed, steps=steps)
+    prefix = reqs[:args.plain_requests]
+    budget = args.hbm_budget_gb * 1e9
+    log(f"served stream: {len(reqs)} requests, {steps} steps each, "
+        f"{sum(r.n * r.steps for r in reqs):.4g} particle-steps, last arrival {reqs[-1].arrival:.1f} s")
+
+    # warm-up: both paths on a few requests of another seed
+    warm = serve.make_requests(count=max(args.warmup, 1), rate=args.rate, seed=args.seed + 1, steps=4)
+    serve.serve_batched(warm, hbm_budget=budget, realtime=False, compensated_sums=sums, graph_steps=args.graph_steps)
+    serve.serve_plain(warm[:2], realtime=False, compensated_sums=sums)
+    log("warm-up done")
+
+    sat = serve.serve_batched(reqs, hbm_budget=budget, realtime=False, compensated_sums=sums,
+                              graph_steps=args.graph_steps, log=log)
+    log(f"batched, saturated: {sat['particle_steps_per_s']:.4g} particle-steps/s over {sat['wall_s']:.1f} s")
+    sat_prefix = serve.serve_batched(prefix, hbm_budget=budget, realtime=False, compensated_sums=sums,
+                                     graph_steps=args.graph_steps)
+    plain_prefix = serve.serve_plain(prefix, realtime=False, compensated_sums=sums, log=log)
+    log(f"first {len(prefix)} requests, saturated: batched {sat_prefix['particle_steps_per_s']:.4g}, "
+        f"plain {plain_prefix['particle_steps_per_s']:.4g} particle-steps/s")
+    live = serve.serve_batched(reqs, hbm_budget=budget, realtime=True, compensated_sums=sums,
+                               graph_steps=args.graph_steps, log=log)
+    log(f"batched, Poisson {args.rate}/s: p50 {live['latency_p50_s'] * 1e3:.1f} ms, "
+        f"p99 {live['latency_p99_s'] * 1e3:.1f} ms")
+    live_plain = None
+    if args.plain_served:
+        live_plain = serve.serve_plain(prefix, realtime=True, compensated_sums=sums, log=log)
+        log(f"plain, Poisson {args.rate}/s, first {len(prefix)}: p50 {live_plain['latency_p50_s'] * 1e3:.1f} ms, "
+            f"p99 {live_plain['latency_p99_s'] * 1e3:.1f} ms")
+
+    # profiled pass: per-pass HIP-event times on a prefix, without graphs
+    prof = serve.serve_batched(reqs[:args.profile_requests], hbm_budget=budget, realtime=False,
+                               compensated_sums=sums, profile=True)
+    stats = prof["kernel_stats"]
+    dominant = max(stats, key=lambda k: stats[k][1])
+    launches, total_ms = stats[dominant]
+    avg_s = total_ms / launches / 1e3
+    bytes_launch = roofline.batch_bytes_per_launch(dominant, prof["particle_steps_by_system"], launches)
+    roof = None
+    if bytes_launch:
+        achieved = bytes_launch / avg_s / 1e9
+        roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.MI355X_HBM_GBS,
+                "unit": "GB/s", "frac": round(achieved / roofline.MI355X_HBM_GBS, 5),
+                "traffic": measured_traffic("served_stream", dominant),
+                "bytes_per_launch": round(bytes_launch), "avg_launch_ms": round(avg_s * 1e3, 5),
+                "sample": f"first {len(reqs[:args.profile_requests])} requests, saturated, no graphs"}
+    pass_ms = {k: round(ms / max(n, 1), 5) for k, (n, ms) in sorted(stats.items(), key=lambda x: -x[1][1])}
+    log("per-pass ms/launch: " + ", ".join(f"{k}={v}" for k, v in pass_ms.items()))
+
+    cpu = None
+    if args.cpu_budget > 0:
+        log("CPU baseline (FP32 oracle) ...")
+        picks = []
+        for kind in serve.KINDS:
+            r = next(q for q in reqs if q.kind == kind)
+            picks.append(r.scene())
+        cpu = cpu_baseline(picks, args.cpu_budget)
+        log(f"CPU baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} threads")
+
+    return {
+        "metric": METRIC, "value": round(sat["particle_steps_per_s"], 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(sat["wall_s"] / max(sat["batch_steps"], 1) * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic (seeded request stream, seed {args.seed}: blocks with seeded jitter / smooth velocity)",
+        "config": {"workload": "served_stream", "requests": len(reqs), "rate_per_s": args.rate,
+                   "mix": "50% fluid / 30% corotational / 20% svk", "particles": "log-uniform 1024-262144",
+                   "steps_per_request": steps, "hbm_budget_gb": args.hbm_budget_gb, "sums": args.sums,
+                   "polar": "fp64", "graph_steps": args.graph_steps, "path": "batched"},
+        "batched_saturated": _brief(sat),
+        "prefix": {"requests": len(prefix), "batched": _brief(sat_prefix), "plain": _brief(plain_prefix),
+                   "speedup": round(sat_prefix["particle_steps_per_s"] / plain_prefix["particle_steps_per_s"], 3)},
+        "served": {"batched": _brief(live), "plain_prefix": _brief(live_plain) if live_plain else None},
+        "error_vs_single_scene": "bitwise identical (tests/test_gpu_batch.py)",
+        "roofline": roof, "pass_ms_per_launch": pass_ms, "cpu_baseline": cpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="steps per request (served stream, default 200) or timed steps (single scene, default 50)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="served_stream", choices=sorted(WORKLOADS))
+    ap.add_argument("--sums", default="kahan-density", choices=sorted(SUMS),
+                    help="neighbour-sum precision (default: Kahan-compensated density sums)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work (0: skip)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a device copy of this many MiB (counter calibration under rocprofv3)")
+    ap.add_argument("--requests", type=int, default=2048, help="served stream: number of requests")
+    ap.add_argument("--rate", type=float, default=40.0, help="served stream: Poisson arrival rate per second")
+    ap.add_argument("--seed", type=int, default=0, help="served stream: seed")
+    ap.add_argument("--plain-requests", type=int, default=192,
+                    help="served stream: prefix run through the single-scene path for comparison")
+    ap.add_argument("--plain-served", type=int, default=1, help="served stream: also serve the prefix one at a time")
+    ap.add_argument("--profile-requests", type=int, default=64, help="served stream: prefix of the profiled pass")
+    ap.add_argument("--hbm-budget-gb", type=float, default=200.0, help="served stream: admission budget")
+    ap.add_argument("--graph-steps", type=int, default=8, help="served stream: batch steps per hipGraph (even)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the engine runs one scene set per device and this build has no "
+              f"multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+    args.steps_set = args.steps is not None
+    if args.steps is None:
+        args.steps = 50
+
+    import sph
+    from sph import bindings
+
+    bindings.set_log_level(3)          # library notes would share stdout with the JSON line
+    sph.init_hip()
+    line = run_served(args) if args.workload == "served_stream" else run_single(args)
     print(json.dumps(line), flush=True)
 
 

@@ -211,6 +211,9 @@ int sph_batch_free(sph_batch *batch);
  * compute their reference configuration (init_original_position,
  * compute_original_density) before their first step. */
 int sph_batch_add(sph_batch *batch, psdata *data, const char *kernel_file_list, unsigned steps, int *scene);
+/* 1 when a scene of `slots` particles and `cells` grid cells (plus one
+ * padding cell) would be admitted now, else 0 (no allocation). */
+int sph_batch_can_admit(sph_batch *batch, unsigned slots, unsigned cells);
 /* Releases a scene's slots and cells (finished or not). */
 int sph_batch_remove(sph_batch *batch, int scene);
 /* Enqueues num_steps batch steps (asynchronous; sph_batch_synchronize or a
@@ -225,6 +228,13 @@ int sph_batch_steps_left(sph_batch *batch, int scene, unsigned *steps_left);
  * `data` (position, velocity, force, density, density0, originalpos,
  * rotation, strain, stress, deformation, body_number). */
 int sph_batch_read(sph_batch *batch, int scene, psdata data, size_t num_fields, const char *const *names);
+/* Per-pass timing with HIP events on the batch's stream, as
+ * set_profiling_hip / get_kernel_stats_hip; steps run without graphs while
+ * profiling is on. */
+int sph_batch_set_profiling(sph_batch *batch, int enabled);
+int sph_batch_reset_kernel_stats(sph_batch *batch);
+int sph_batch_get_kernel_stats(sph_batch *batch, int capacity, const char **names, unsigned long long *launches,
+                               double *total_ms);
 /* out[0..7]: resident scenes, scenes with steps left, used slots, used
  * cells, slot capacity, cell capacity, graph captures, graph launches. */
 int sph_batch_info(sph_batch *batch, unsigned long long *out);

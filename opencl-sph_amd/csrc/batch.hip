@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -185,6 +186,11 @@ using sphk::SlotState;
 using sphk::StepParams;
 using sphk::kNoScene;
 
+struct BatchTiming {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
 struct sph_batch {
     hipStream_t stream = nullptr;
     int device = 0;
@@ -215,6 +221,13 @@ struct sph_batch {
     unsigned setup_pending = 0;
     bool layout_dirty = true;              // segs / used extents changed since the last upload
     unsigned long long layout_version = 1;
+
+    // per-pass timing
+    bool profiling = false;
+    std::map<std::string, std::pair<unsigned long long, double>> stats;   // launches, total ms
+    std::vector<BatchTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<std::string> stat_names;
 
     // graph replay of K steps
     unsigned graph_steps = 0;
@@ -254,6 +267,11 @@ void release(sph_batch *b) {
     if (!b) return;
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->graph) hipGraphExecDestroy(b->graph);
+    for (BatchTiming &t : b->pending) {
+        hipEventDestroy(t.start);
+        hipEventDestroy(t.stop);
+    }
+    for (hipEvent_t e : b->event_pool) hipEventDestroy(e);
     for (void *p : b->allocations) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -311,35 +329,90 @@ unsigned systems_present(const sph_batch *b) {
     return m;
 }
 
-#define LAUNCH_TRY(expr)                                                                          \
-    do {                                                                                          \
-        hipError_t err_ = (expr);                                                                 \
-        if (err_ != hipSuccess) return fail("%s failed: %s", #expr, hipGetErrorString(err_));     \
-    } while (0)
+// ---- per-pass timing (HIP events on the batch stream, eager steps only) ----
+hipEvent_t take_event(sph_batch *b) {
+    if (!b->event_pool.empty()) {
+        hipEvent_t e = b->event_pool.back();
+        b->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+
+int drain_timings(sph_batch *b) {
+    if (b->pending.empty()) return 0;
+    SPH_HIP_TRY(hipEventSynchronize(b->pending.back().stop));
+    for (BatchTiming &t : b->pending) {
+        float ms = 0.0f;
+        SPH_HIP_TRY(hipEventElapsedTime(&ms, t.start, t.stop));
+        auto &st = b->stats[t.name];
+        st.first += 1;
+        st.second += ms;
+        b->event_pool.push_back(t.start);
+        b->event_pool.push_back(t.stop);
+    }
+    b->pending.clear();
+    return 0;
+}
+
+// Runs `launch` (which returns the launch status); brackets it with events
+// when profiling.
+template <typename F>
+int timed(sph_batch *b, const char *name, F &&launch) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (b->profiling) {
+        e0 = take_event(b);
+        e1 = take_event(b);
+        if (!e0 || !e1) return fail("hipEventCreate failed");
+        SPH_HIP_TRY(hipEventRecord(e0, b->stream));
+    }
+    const hipError_t e = launch();
+    if (e != hipSuccess) return fail("%s launch failed: %s", name, hipGetErrorString(e));
+    if (b->profiling) {
+        SPH_HIP_TRY(hipEventRecord(e1, b->stream));
+        b->pending.push_back(BatchTiming{name, e0, e1});
+        if (b->pending.size() >= 4096) return drain_timings(b);
+    }
+    return 0;
+}
 
 // Binning of every scene with steps left; swaps the state buffers.
 int enqueue_bins(sph_batch *b) {
     hipStream_t s = b->stream;
     const sphk::BatchTable t = table(b);
     const sphk::SceneTable all{t, sphk::kAllScenes};
-    const SlotState &a = b->st[b->cur];
-    const SlotState &n = b->st[b->cur ^ 1];
-    LAUNCH_TRY(sphk::launch_fill_u32(b->gridcount, b->used_cells, 0u, s));
-    hipLaunchKernelGGL(sphk::k_batch_padding, sphk::blocks(b->segs.size()), dim3(sphk::kBlock), 0, s, b->d_segs,
-                       static_cast<unsigned>(b->segs.size()), b->d_scenes, b->d_steps_left, b->gridcount);
-    hipLaunchKernelGGL(sphk::k_batch_key_count, sphk::blocks(b->used_slots), dim3(sphk::kBlock), 0, s, all, a.pos,
-                       b->bins.key, b->bins.local, b->gridcount);
-    LAUNCH_TRY(hipGetLastError());
-    LAUNCH_TRY(sphk::launch_exclusive_scan(b->gridcount, b->used_cells, b->celloffset, b->bins.tile_sums, s));
-    hipLaunchKernelGGL(sphk::k_batch_scatter, sphk::blocks(b->used_slots), dim3(sphk::kBlock), 0, s, all,
-                       b->bins.key, b->bins.local, b->celloffset, a.id, b->bins.tmp_id, b->bins.tmp_src,
-                       b->bins.tmp_key);
-    hipLaunchKernelGGL(sphk::k_batch_rank, sphk::blocks(b->used_slots), dim3(sphk::kBlock), 0, s, all,
-                       b->bins.tmp_id, b->bins.tmp_src, b->bins.tmp_key, b->celloffset, b->gridcount, b->perm,
-                       b->gridcell);
-    hipLaunchKernelGGL(sphk::k_batch_gather, sphk::blocks(b->used_slots), dim3(sphk::kBlock), 0, s, all, b->perm,
-                       a, n);
-    LAUNCH_TRY(hipGetLastError());
+    const SlotState a = b->st[b->cur];
+    const SlotState n = b->st[b->cur ^ 1];
+    const dim3 grid = sphk::blocks(b->used_slots), block(sphk::kBlock);
+    const unsigned nsegs = static_cast<unsigned>(b->segs.size());
+    if (timed(b, "batch_key_count", [&] {
+            hipError_t e = sphk::launch_fill_u32(b->gridcount, b->used_cells, 0u, s);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(sphk::k_batch_padding, sphk::blocks(nsegs), block, 0, s, b->d_segs, nsegs,
+                               b->d_scenes, b->d_steps_left, b->gridcount);
+            hipLaunchKernelGGL(sphk::k_batch_key_count, grid, block, 0, s, all, a.pos, b->bins.key, b->bins.local,
+                               b->gridcount);
+            return hipGetLastError();
+        }) ||
+        timed(b, "batch_scan", [&] {
+            return sphk::launch_exclusive_scan(b->gridcount, b->used_cells, b->celloffset, b->bins.tile_sums, s);
+        }) ||
+        timed(b, "batch_scatter", [&] {
+            hipLaunchKernelGGL(sphk::k_batch_scatter, grid, block, 0, s, all, b->bins.key, b->bins.local,
+                               b->celloffset, a.id, b->bins.tmp_id, b->bins.tmp_src, b->bins.tmp_key);
+            return hipGetLastError();
+        }) ||
+        timed(b, "batch_rank", [&] {
+            hipLaunchKernelGGL(sphk::k_batch_rank, grid, block, 0, s, all, b->bins.tmp_id, b->bins.tmp_src,
+                               b->bins.tmp_key, b->celloffset, b->gridcount, b->perm, b->gridcell);
+            return hipGetLastError();
+        }) ||
+        timed(b, "batch_gather", [&] {
+            hipLaunchKernelGGL(sphk::k_batch_gather, grid, block, 0, s, all, b->perm, a, n);
+            return hipGetLastError();
+        }))
+        return -1;
     b->cur ^= 1;
     return 0;
 }
@@ -355,29 +428,46 @@ int enqueue_step(sph_batch *b, unsigned systems, bool setup) {
     const sphk::SceneFilter fluid{1u << sphk::kFluid, 0u}, solids{solid_mask, 0u};
     const sphk::SceneFilter corot{1u << sphk::kCorotational, 0u}, svk{1u << sphk::kSvk, 0u};
     const sphk::SceneFilter setup_sel{solid_mask, sphk::kSceneSetup};
-    if (setup) LAUNCH_TRY(sphk::launch_batch_init_original_position(t, setup_sel, field_ptrs(b), s));
+    if (setup && timed(b, "init_original_position", [&] {
+            return sphk::launch_batch_init_original_position(t, setup_sel, field_ptrs(b), s);
+        }))
+        return -1;
     if (enqueue_bins(b)) return -1;
     const sphk::FieldPtrs f = field_ptrs(b);
-    if (setup) LAUNCH_TRY(sphk::launch_batch_original_density(t, setup_sel, f, kahan, s));
-    if (systems & (1u << sphk::kFluid)) {
-        LAUNCH_TRY(sphk::launch_batch_density_listed(t, fluid, f, b->pairs, kahan, s));
-        LAUNCH_TRY(sphk::launch_batch_forces_fluids_listed(t, fluid, f, b->pairs, kahan, s));
-    }
-    if (systems & solid_mask) LAUNCH_TRY(sphk::launch_batch_density(t, solids, f, kahan, s));
-    if (systems & (1u << sphk::kCorotational)) {
-        LAUNCH_TRY(sphk::launch_batch_rotations_and_strains(t, corot, f, kahan, fp64, s));
-        LAUNCH_TRY(sphk::launch_batch_stresses_corotational(t, corot, f, s));
-        LAUNCH_TRY(sphk::launch_batch_forces_solids(t, corot, f, true, kahan, s));
-    }
-    if (systems & (1u << sphk::kSvk)) {
-        LAUNCH_TRY(sphk::launch_batch_deformations_svk(t, svk, f, kahan, fp64, s));
-        LAUNCH_TRY(sphk::launch_batch_stresses_svk(t, svk, f, s));
-        LAUNCH_TRY(sphk::launch_batch_forces_solids(t, svk, f, false, kahan, s));
-    }
-    LAUNCH_TRY(sphk::launch_batch_step_forward(t, sphk::kAllScenes, f, s));
-    hipLaunchKernelGGL(sphk::k_batch_tick, sphk::blocks(b->scene_cap), dim3(sphk::kBlock), 0, s, b->scene_cap,
-                       b->d_steps_left);
-    LAUNCH_TRY(hipGetLastError());
+    if (setup && timed(b, "compute_original_density",
+                       [&] { return sphk::launch_batch_original_density(t, setup_sel, f, kahan, s); }))
+        return -1;
+    if ((systems & (1u << sphk::kFluid)) &&
+        (timed(b, "compute_density_fluid",
+               [&] { return sphk::launch_batch_density_listed(t, fluid, f, b->pairs, kahan, s); }) ||
+         timed(b, "compute_forces_fluids",
+               [&] { return sphk::launch_batch_forces_fluids_listed(t, fluid, f, b->pairs, kahan, s); })))
+        return -1;
+    if ((systems & solid_mask) &&
+        timed(b, "compute_density_solid", [&] { return sphk::launch_batch_density(t, solids, f, kahan, s); }))
+        return -1;
+    if ((systems & (1u << sphk::kCorotational)) &&
+        (timed(b, "compute_rotations_and_strains",
+               [&] { return sphk::launch_batch_rotations_and_strains(t, corot, f, kahan, fp64, s); }) ||
+         timed(b, "compute_stresses_corotational",
+               [&] { return sphk::launch_batch_stresses_corotational(t, corot, f, s); }) ||
+         timed(b, "compute_forces_solids_corotational",
+               [&] { return sphk::launch_batch_forces_solids(t, corot, f, true, kahan, s); })))
+        return -1;
+    if ((systems & (1u << sphk::kSvk)) &&
+        (timed(b, "compute_deformations",
+               [&] { return sphk::launch_batch_deformations_svk(t, svk, f, kahan, fp64, s); }) ||
+         timed(b, "compute_stresses_svk", [&] { return sphk::launch_batch_stresses_svk(t, svk, f, s); }) ||
+         timed(b, "compute_forces_solids_svk",
+               [&] { return sphk::launch_batch_forces_solids(t, svk, f, false, kahan, s); })))
+        return -1;
+    if (timed(b, "step_forward", [&] { return sphk::launch_batch_step_forward(t, sphk::kAllScenes, f, s); }) ||
+        timed(b, "batch_tick", [&] {
+            hipLaunchKernelGGL(sphk::k_batch_tick, sphk::blocks(b->scene_cap), dim3(sphk::kBlock), 0, s,
+                               b->scene_cap, b->d_steps_left);
+            return hipGetLastError();
+        }))
+        return -1;
     return 0;
 }
 
@@ -387,6 +477,14 @@ void reindex(sph_batch *b) {
         if (b->segs[j].scene != kNoScene) b->seg_of[b->segs[j].scene] = static_cast<unsigned>(j);
     b->layout_dirty = true;
     b->layout_version += 1;
+}
+
+// Whether alloc_segment would find room (a free segment or the unused tail).
+bool segment_fits(const sph_batch *b, unsigned need_slots, unsigned need_cells) {
+    for (const Segment &g : b->segs)
+        if (g.scene == kNoScene && g.cap >= need_slots && g.cellcap >= need_cells) return true;
+    return static_cast<size_t>(b->used_slots) + need_slots <= b->slot_cap &&
+           static_cast<size_t>(b->used_cells) + need_cells <= b->cell_cap;
 }
 
 int alloc_segment(sph_batch *b, unsigned need_slots, unsigned need_cells, unsigned scene, unsigned *seg_index) {
@@ -656,6 +754,13 @@ extern "C" int sph_batch_add(sph_batch *b, psdata *data, const char *kernel_file
     return 0;
 }
 
+extern "C" int sph_batch_can_admit(sph_batch *b, unsigned slots, unsigned cells) {
+    if (!b) return fail("null batch");
+    bool entry = false;
+    for (unsigned s = 0; s < b->scene_cap && !entry; ++s) entry = b->seg_of[s] == kNoScene;
+    return entry && slots > 0 && segment_fits(b, slots, cells) ? 1 : 0;
+}
+
 extern "C" int sph_batch_remove(sph_batch *b, int scene) {
     if (!b) return fail("null batch");
     if (scene < 0 || static_cast<unsigned>(scene) >= b->scene_cap || b->seg_of[scene] == kNoScene)
@@ -698,7 +803,7 @@ extern "C" int sph_batch_step(sph_batch *b, unsigned num_steps) {
             // the upload reads host memory that the next step may change
             SPH_HIP_TRY(hipStreamSynchronize(b->stream));
             done += 1;
-        } else if (b->graph_steps > 0 && num_steps - done >= b->graph_steps) {
+        } else if (b->graph_steps > 0 && !b->profiling && num_steps - done >= b->graph_steps) {
             if (!b->graph || b->graph_version != b->layout_version || b->graph_cur != b->cur) {
                 if (b->graph) {
                     SPH_HIP_TRY(hipGraphExecDestroy(b->graph));
@@ -783,6 +888,38 @@ extern "C" int sph_batch_read(sph_batch *b, int scene, psdata data, size_t num_f
         SPH_HIP_TRY(hipStreamSynchronize(b->stream));
     }
     return 0;
+}
+
+extern "C" int sph_batch_set_profiling(sph_batch *b, int enabled) {
+    if (!b) return fail("null batch");
+    if (!enabled && drain_timings(b)) return -1;
+    b->profiling = enabled != 0;
+    return 0;
+}
+
+extern "C" int sph_batch_reset_kernel_stats(sph_batch *b) {
+    if (!b) return fail("null batch");
+    if (drain_timings(b)) return -1;
+    b->stats.clear();
+    return 0;
+}
+
+extern "C" int sph_batch_get_kernel_stats(sph_batch *b, int capacity, const char **names,
+                                          unsigned long long *launches, double *total_ms) {
+    if (!b) return fail("null batch");
+    if (drain_timings(b)) return -1;
+    b->stat_names.clear();
+    for (auto &kv : b->stats) b->stat_names.push_back(kv.first);
+    int k = 0;
+    for (auto &kv : b->stats) {
+        if (k < capacity) {
+            if (names) names[k] = b->stat_names[k].c_str();
+            if (launches) launches[k] = kv.second.first;
+            if (total_ms) total_ms[k] = kv.second.second;
+        }
+        ++k;
+    }
+    return k;
 }
 
 // out[0..7]: live scenes, scenes with steps left, used slots, used cells,

@@ -57,6 +57,10 @@ class Batch:
         res = [int(abs(float(gb[2 * a + 1] - gb[2 * a])) / h) for a in range(3)]
         return scene.n, res[0] * res[1] * res[2] + 1
 
+    def can_admit(self, scene):
+        """Whether `scene` would be admitted now."""
+        return self._batch.can_admit(*self.footprint(scene))
+
     def add(self, scene, steps=None):
         """Admit `scene` for `steps` steps (default scene.steps).  Returns its
         handle, or None when the pool has no room for it now."""
@@ -100,6 +104,15 @@ class Batch:
 
     def info(self):
         return self._batch.info()
+
+    def set_profiling(self, enabled=True):
+        self._batch.set_profiling(enabled)
+
+    def reset_kernel_stats(self):
+        self._batch.reset_kernel_stats()
+
+    def kernel_stats(self):
+        return self._batch.kernel_stats()
 
     @property
     def resident(self):

@@ -234,12 +234,30 @@ _batch_create = _fn("sph_batch_create", [c_uint, c_uint, c_uint, c_int, c_int, P
 _batch_free = _fn("sph_batch_free", [c_void_p], c_int)
 _batch_add = _fn("sph_batch_add", [c_void_p, POINTER(psdata), c_char_p, c_uint, POINTER(c_int)], c_int)
 _batch_remove = _fn("sph_batch_remove", [c_void_p, c_int], c_int)
+_batch_can_admit = _fn("sph_batch_can_admit", [c_void_p, c_uint, c_uint], c_int)
 _batch_step = _fn("sph_batch_step", [c_void_p, c_uint], c_int)
 _batch_graph = _fn("sph_batch_set_graph_steps", [c_void_p, c_uint], c_int)
 _batch_sync = _fn("sph_batch_synchronize", [c_void_p], c_int)
 _batch_left = _fn("sph_batch_steps_left", [c_void_p, c_int, POINTER(c_uint)], c_int)
 _batch_read = _fn("sph_batch_read", [c_void_p, c_int, psdata, c_size_t, POINTER(c_char_p)], c_int)
 _batch_info = _fn("sph_batch_info", [c_void_p, POINTER(c_ulonglong)], c_int)
+_batch_prof = _fn("sph_batch_set_profiling", [c_void_p, c_int], c_int)
+_batch_reset_stats = _fn("sph_batch_reset_kernel_stats", [c_void_p], c_int)
+_batch_stats = _fn("sph_batch_get_kernel_stats", [c_void_p, c_int, POINTER(c_char_p), POINTER(c_ulonglong),
+                                                  POINTER(c_double)], c_int)
+
+
+def _stats(fn, handle):
+    count = fn(handle, 0, None, None, None)
+    if count < 0:
+        check(-1)
+    names = (c_char_p * max(count, 1))()
+    launches = (c_ulonglong * max(count, 1))()
+    total = (c_double * max(count, 1))()
+    count = fn(handle, count, names, launches, total)
+    if count < 0:
+        check(-1)
+    return {names[k].decode(): (int(launches[k]), float(total[k])) for k in range(count)}
 
 
 def batch_device_bytes(slots, cells, scenes):
@@ -271,6 +289,14 @@ class SphBatch:
         check(rc)
         return scene.value
 
+    def can_admit(self, slots, cells):
+        """Whether a scene of `slots` particles and `cells` cells (padding
+        cell included) fits now."""
+        rc = _batch_can_admit(self.handle, int(slots), int(cells))
+        if rc < 0:
+            check(rc)
+        return rc == 1
+
     def remove(self, scene):
         check(_batch_remove(self.handle, int(scene)))
 
@@ -291,6 +317,16 @@ class SphBatch:
     def read(self, scene, data, names):
         arr = _names_array(names)
         check(_batch_read(self.handle, int(scene), data, len(names), arr))
+
+    def set_profiling(self, enabled=True):
+        check(_batch_prof(self.handle, int(bool(enabled))))
+
+    def reset_kernel_stats(self):
+        check(_batch_reset_stats(self.handle))
+
+    def kernel_stats(self):
+        """{pass name: (launches, total milliseconds)} from HIP events."""
+        return _stats(_batch_stats, self.handle)
 
     def info(self):
         out = (c_ulonglong * 8)()

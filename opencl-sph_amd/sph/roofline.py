@@ -24,6 +24,46 @@ _FLUID_KERNELS = {
 }
 
 
+# Batch passes: (systems the pass runs for, bytes per active particle, bytes
+# per grid cell).  Every pass also reads each slot's scene entry (4 B), not
+# counted, as padding slots and scene tables are overhead of the batch.
+# The batch writes only `force` from the solid force passes (no exported
+# components), and gathers the persistent state of each system in binning.
+BATCH_PASSES = {
+    "batch_key_count": (("fluid", "corotational", "svk"), 20, 8),
+    "batch_scan": ((), 0, 8),
+    "batch_scatter": (("fluid", "corotational", "svk"), 28, 0),
+    "batch_rank": (("fluid", "corotational", "svk"), 20, 0),
+    "compute_density_fluid": (("fluid",), 24, 8),
+    "compute_forces_fluids": (("fluid",), 44, 8),
+    "compute_density_solid": (("corotational", "svk"), 16, 8),
+    "compute_rotations_and_strains": (("corotational",), 88, 8),
+    "compute_stresses_corotational": (("corotational",), 48, 0),
+    "compute_forces_solids_corotational": (("corotational",), 120, 8),
+    "compute_deformations": (("svk",), 60, 8),
+    "compute_stresses_svk": (("svk",), 60, 0),
+    "compute_forces_solids_svk": (("svk",), 84, 8),
+    "step_forward": (("fluid", "corotational", "svk"), 72, 0),
+}
+# gathered words per particle in binning: pos, vel, id (+ opos, density0,
+# body; + rotation, strain for corotational), read and written, + perm
+GATHER_BYTES = {"fluid": 2 * 7 * 4 + 4, "corotational": 2 * 27 * 4 + 4, "svk": 2 * 12 * 4 + 4}
+
+
+def batch_bytes_per_launch(name, particle_steps_by_system, launches, cell_steps=0):
+    """Average algorithmic bytes of one launch of batch pass `name`, from the
+    particle-steps each system ran while the pass was timed."""
+    if name == "batch_gather":
+        total = sum(GATHER_BYTES[k] * v for k, v in particle_steps_by_system.items())
+        return total / launches if launches else None
+    m = BATCH_PASSES.get(name)
+    if m is None or not launches:
+        return None
+    systems, per_particle, per_cell = m
+    total = per_particle * sum(particle_steps_by_system.get(k, 0) for k in systems) + per_cell * cell_steps
+    return total / launches
+
+
 def kernel_model(name):
     return _FLUID_KERNELS.get(name)
 

@@ -12,7 +12,9 @@ The workloads are:
     dam_break(...)       weakly compressible fluid block
     cantilever(...)      corotational beam with its x = 0 face clamped
     svk_cube(...)        SVK cube with a smooth random initial velocity
-    baseline_config(k)   the five benchmark workloads (k = 0..3 here)
+    svk_block(...)       the same for a rectangular block
+    request_scene(...)   a block of any of the three for the served stream
+    baseline_config(k)   the single-scene benchmark workloads (k = 0..3)
 
 Two optional conf fields extend the fluid model: `eos_tait` (Tait equation
 of state p = k((rho/rho0)^7 - 1)) and `mass_consistent` (multiply the fluid
@@ -207,13 +209,18 @@ def smooth_velocity(pos, amplitude, seed, box):
 
 def svk_cube(nside=64, dx=0.01, h=0.02, youngs=5e5, poisson=0.35, rho=1100.0, dt=2e-5, viscosity=4e-4,
              amplitude=0.05, seed=0, steps=200, name="svk_cube"):
-    pts, _ = lattice(nside, nside, nside, dx)
+    return svk_block(nside, nside, nside, dx, h, youngs, poisson, rho, dt, viscosity, amplitude, seed, steps, name)
+
+
+def svk_block(nx, ny, nz, dx=0.01, h=0.02, youngs=5e5, poisson=0.35, rho=1100.0, dt=2e-5, viscosity=4e-4,
+              amplitude=0.05, seed=0, steps=200, name="svk_block"):
+    pts, _ = lattice(nx, ny, nz, dx)
     pos = pts.astype(np.float32)
     n = len(pos)
     lo, hi = pos.min(0) - 4 * h, pos.max(0) + 4 * h
     gb = [float(lo[0]), float(hi[0]), float(lo[1]), float(hi[1]), float(lo[2]), float(hi[2])]
     mass = rho * dx ** 3
-    vel = smooth_velocity(pts, amplitude, seed, box=[nside * dx] * 3).astype(np.float32)
+    vel = smooth_velocity(pts, amplitude, seed, box=[nx * dx, ny * dx, nz * dx]).astype(np.float32)
     spec = solid_spec(n, mass, dt, h, gb, viscosity, youngs, poisson, gravity=(0, 0, 0), svk=True)
     fields = {"position": pos, "velocity": vel, "n": np.array([n], np.uint32)}
     params = dict(n=n, gridbounds=gb, h=h, mass=float(np.float32(mass)), dt=dt, viscosity=viscosity,
@@ -234,6 +241,36 @@ def baseline_config(k, seed=0):
         return dam_break(256, 256, 64, 0.005, 0.01, 2.5e-5, seed=seed, gridbounds=(0, 2, 0, 2, 0, 1), steps=50,
                          name="dam_break_4m")
     raise ValueError(f"no workload {k}")
+
+
+def block_dims(n, rng):
+    """Rectangular block of about n particles with random aspect: log-aspect
+    ratios uniform in [-ln 2, ln 2] per axis, the last axis adjusted so the
+    product lands as close to n as integers allow."""
+    a = np.exp(rng.uniform(-np.log(2.0), np.log(2.0), size=3))
+    a /= np.prod(a) ** (1.0 / 3.0)
+    side = n ** (1.0 / 3.0)
+    nx = max(2, int(round(side * a[0])))
+    ny = max(2, int(round(side * a[1])))
+    nz = max(2, int(round(n / (nx * ny))))
+    return nx, ny, nz
+
+
+def request_scene(kind, nx, ny, nz, seed, steps=200, name=None):
+    """A served-stream scene: a block of the given lattice size with the
+    parameters of the 16k dam break (fluid), the cantilever (corotational,
+    x = 0 face clamped) or the SVK cube (svk)."""
+    name = name or f"{kind}_{nx}x{ny}x{nz}"
+    if kind == "fluid":
+        h, dx = 0.04, 0.02
+        top = [h + (m - 1) * dx + 2 * h for m in (nx, ny, nz)]
+        return dam_break(nx, ny, nz, dx, h, 1e-4, seed=seed, gridbounds=(0, top[0], 0, top[1], 0, top[2]),
+                         steps=steps, name=name)
+    if kind == "corotational":
+        return cantilever(nx, ny, nz, steps=steps, name=name)
+    if kind == "svk":
+        return svk_block(nx, ny, nz, seed=seed, steps=steps, name=name)
+    raise ValueError(f"unknown system {kind}")
 
 
 def build(scene, setup=True):

@@ -1,0 +1,211 @@
+"""Served scene stream: requests arrive over time, are admitted into a device
+batch as slots and memory free up, run their steps together with whatever
+else is resident, and are read back as soon as they finish.
+
+    requests = make_requests(count=2048, rate=40.0, seed=0)
+    report = serve_batched(requests, hbm_budget=200e9)
+    report = serve_plain(requests)          # one scene at a time, for comparison
+
+A request is a seeded scene description (system, block size, seed, steps);
+its scene is generated when it is admitted.  Latency is the time from a
+request's arrival to its results being in host memory.  With realtime=False
+every request is available at time 0 (a saturated queue), which measures
+throughput alone.
+
+There is no counterpart in the reference (which steps one psdata at a time
+from Python, scripts/sph/systems.py:148-189); the per-scene step sequence is
+the reference's, through the batch (sph.batch) or the single-scene path
+(scenes.build).
+"""
+import dataclasses
+import time
+from collections import deque
+
+import numpy as np
+
+from . import scenes
+
+KINDS = ("fluid", "corotational", "svk")
+
+
+@dataclasses.dataclass
+class Request:
+    index: int
+    arrival: float          # seconds after the start of the stream
+    kind: str
+    dims: tuple
+    seed: int
+    steps: int
+
+    @property
+    def n(self):
+        return int(np.prod(self.dims))
+
+    def scene(self):
+        return scenes.request_scene(self.kind, *self.dims, seed=self.seed, steps=self.steps,
+                                    name=f"request_{self.index}")
+
+
+def make_requests(count=2048, rate=40.0, seed=0, mix=(0.5, 0.3, 0.2), nmin=1024, nmax=262144, steps=200):
+    """Seeded request stream: exponential inter-arrival times at `rate` per
+    second, system drawn from `mix`, particle count log-uniform in
+    [nmin, nmax] as a block of random aspect."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        kind = KINDS[int(rng.choice(3, p=np.asarray(mix) / np.sum(mix)))]
+        n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        dims = scenes.block_dims(n, rng)
+        out.append(Request(i, t, kind, dims, int(rng.integers(0, 2**31 - 1)), steps))
+    return out
+
+
+def _summary(name, requests, done, t_start, t_end, extra):
+    lat = np.array([done[r.index]["latency"] for r in requests])
+    work = float(sum(r.n * r.steps for r in requests))
+    by_system = {k: float(sum(r.n * r.steps for r in requests if r.kind == k)) for k in KINDS}
+    wall = t_end - t_start
+    rep = {
+        "path": name, "requests": len(requests), "particle_steps": work, "particle_steps_by_system": by_system,
+        "wall_s": wall,
+        "particle_steps_per_s": work / wall if wall > 0 else 0.0,
+        "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+        "latency_mean_s": float(lat.mean()), "latency_max_s": float(lat.max()),
+    }
+    rep.update(extra)
+    return rep
+
+
+def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entries=4096, graph_steps=8,
+                  compensated_sums=1, polar_fp64=True, realtime=True, on_result=None, fields=("position",),
+                  window=4096, profile=False, log=None):
+    """Continuous batching over one device pool.
+
+    The pool is sized to the HBM budget (or to `slots` / `cells` when given,
+    within the budget).  Each round of the loop admits every arrived request
+    that fits (in arrival order, looking up to `window` requests past ones
+    that do not fit yet), runs
+    `graph_steps` batch steps, then reads back and releases the scenes that
+    finished.  Scene generation for waiting requests overlaps the device's
+    steps.  on_result(request, {field: array}) receives each result."""
+    from .batch import Batch
+
+    slots = slots or 32 << 20
+    cells = cells or 2 * slots
+    while Batch.device_bytes(slots, cells, scene_entries) > hbm_budget and slots > 1024:
+        slots //= 2
+        cells //= 2
+    pool_bytes = Batch.device_bytes(slots, cells, scene_entries)
+    batch = Batch(slots, cells, scene_entries, compensated_sums=compensated_sums, polar_fp64=polar_fp64,
+                  graph_steps=0 if profile else graph_steps)
+    if profile:
+        batch.set_profiling(True)
+    stats = {}
+    chunk = max(graph_steps, 1)
+    pending = deque(sorted(requests, key=lambda r: r.arrival))
+    waiting = []                  # [request, scene or None]
+    running = {}                  # handle -> (request, admitted at)
+    done = {}
+    busy_steps = 0
+    peak_slots = peak_scenes = 0
+    t0 = time.perf_counter()
+    clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
+    try:
+        while pending or waiting or running:
+            now = clock()
+            while pending and pending[0].arrival <= now:
+                waiting.append([pending.popleft(), None])
+            # admission: arrival order, looking past requests that do not fit
+            # yet within a window of the queue
+            admitted = set()
+            misses = 0
+            for k, item in enumerate(waiting[:window]):
+                req, scene = item
+                if scene is None:
+                    scene = item[1] = req.scene()
+                if not batch.can_admit(scene):
+                    misses += 1
+                    if misses >= 4:        # the pool is close to full
+                        break
+                    continue
+                handle = batch.add(scene, req.steps)
+                if handle is None:
+                    continue
+                running[handle] = (req, clock() if realtime else 0.0)
+                admitted.add(k)
+            waiting = [item for k, item in enumerate(waiting) if k not in admitted]
+            if not running:
+                if pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.05)))
+                continue
+            info = batch.info()
+            peak_slots = max(peak_slots, info["used_slots"])
+            peak_scenes = max(peak_scenes, info["scenes"])
+            batch.step(chunk)
+            busy_steps += chunk
+            # generate scenes of requests that arrived meanwhile while the
+            # device works
+            now = clock()
+            while pending and pending[0].arrival <= now:
+                waiting.append([pending.popleft(), None])
+            for item in waiting[:8]:
+                if item[1] is None:
+                    item[1] = item[0].scene()
+            batch.synchronize()
+            for handle in [h for h in running if batch.steps_left(h) == 0]:
+                req, _ = running.pop(handle)
+                result = batch.read(handle, *fields)
+                batch.remove(handle)
+                t_done = time.perf_counter() - t0
+                done[req.index] = {"latency": t_done - (req.arrival if realtime else 0.0), "finished": t_done}
+                if on_result:
+                    on_result(req, result)
+            if log and busy_steps % (chunk * 100) == 0:
+                log(f"batched: {len(done)}/{len(requests)} done, {len(running)} resident, "
+                    f"{len(waiting)} waiting, t={time.perf_counter() - t0:.1f}s")
+        t_end = time.perf_counter() - t0
+        info = batch.info()
+        if profile:
+            stats = batch.kernel_stats()
+    finally:
+        batch.free()
+    t_first = requests[0].arrival if realtime else 0.0
+    return _summary("batched", requests, done, min(t_first, t_end), t_end,
+                    {"pool_bytes": pool_bytes, "pool_slots": slots, "pool_cells": cells,
+                     "peak_resident_slots": peak_slots, "peak_resident_scenes": peak_scenes,
+                     "batch_steps": busy_steps, "graph_captures": info["graph_captures"],
+                     "graph_launches": info["graph_launches"], "graph_steps": graph_steps,
+                     "kernel_stats": stats})
+
+
+def serve_plain(requests, compensated_sums=1, polar_fp64=True, realtime=True, on_result=None, fields=("position",),
+                log=None):
+    """One scene at a time through the single-scene path (its own psdata and
+    device handle, set-up, steps, readback, release), in arrival order."""
+    done = {}
+    t0 = time.perf_counter()
+    for k, req in enumerate(sorted(requests, key=lambda r: r.arrival)):
+        if realtime:
+            wait = req.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+        scene = req.scene()
+        system = scenes.build(scene, setup=False)
+        system.psdata_opencl.set_precision(compensated_sums, polar_fp64)
+        if scene.system in ("corotational", "svk"):
+            system.setup()
+        system.run(req.steps)
+        system.sync_fields_device_to_host(*fields)
+        result = {f: np.array(system.get_fields(f)[0][:scene.n]) for f in fields}
+        system.free()
+        t_done = time.perf_counter() - t0
+        done[req.index] = {"latency": t_done - (req.arrival if realtime else 0.0), "finished": t_done}
+        if on_result:
+            on_result(req, result)
+        if log and (k + 1) % 100 == 0:
+            log(f"plain: {k + 1}/{len(requests)} done, t={t_done:.1f}s")
+    t_end = time.perf_counter() - t0
+    t_first = requests[0].arrival if realtime else 0.0
+    return _summary("plain", requests, done, min(t_first, t_end), t_end, {})

@@ -128,3 +128,23 @@ def test_batch_rejects_bad_scenes():
             batch.add(bad)                                             # no 'common'
     finally:
         batch.free()
+
+
+def test_served_stream_same_alone_and_in_mix():
+    """Requests served by the continuous-batching loop (arrivals while others
+    run, graph replay) give the same bits as the same requests served one at
+    a time through the single-scene path."""
+    from sph import serve
+    reqs = serve.make_requests(count=12, rate=400.0, seed=3, nmin=300, nmax=3000, steps=12)
+    say("served stream: " + ", ".join(f"{r.kind}:{r.n}" for r in reqs))
+    got, want = {}, {}
+    fields = ("position", "velocity")
+    rep = serve.serve_batched(reqs, slots=1 << 16, cells=1 << 18, scene_entries=16, graph_steps=4, realtime=True,
+                              compensated_sums=1, fields=fields, on_result=lambda r, res: got.__setitem__(r.index, res))
+    say(f"  batched: {rep['batch_steps']} batch steps, peak {rep['peak_resident_scenes']} scenes resident, "
+        f"graph launches {rep['graph_launches']}")
+    serve.serve_plain(reqs, realtime=False, compensated_sums=1, fields=fields,
+                      on_result=lambda r, res: want.__setitem__(r.index, res))
+    assert sorted(got) == sorted(want) == list(range(len(reqs)))
+    for r in reqs:
+        check_same(f"request {r.index} ({r.kind}, {r.n})", got[r.index], want[r.index])
