@@ -212,6 +212,14 @@ struct sph_batch {
     unsigned *d_steps_left = nullptr, *body_dynamic = nullptr;
     Segment *d_segs = nullptr;
 
+    // per-system chunk maps (fluid, corotational, svk, both solids): the
+    // 64-slot chunks of the running scenes of each, rebuilt on layout changes
+    static constexpr int kMaps = 4;
+    unsigned *d_chunk_slot[kMaps] = {}, *d_chunk_end[kMaps] = {};
+    unsigned nchunks[kMaps] = {};
+    size_t max_chunks = 0;
+    unsigned long long maps_version = 0;
+
     // host mirror
     std::vector<StepParams> scenes;
     std::vector<unsigned> steps_left;
@@ -256,10 +264,11 @@ size_t pool_bytes(size_t slots, size_t cells, size_t scenes) {
                             + 3 * 4                                    // gridcell, scene_of, perm
                             + 5 * 4                                    // bin scratch
                             + (sphk::kListCap + 2) * 4                 // pair cache
-                            + 9 * 4;                                   // readback staging
+                            + 9 * 4                                    // readback staging
+                            + 1;                                       // chunk maps (4 x 8 B per 64 slots)
     const size_t per_cell = 2 * 4;
     const size_t tiles = (cells + sphk::kScanTile - 1) / sphk::kScanTile + 1;
-    const size_t per_scene = sizeof(StepParams) + 4 + sphk::kBodiesPerScene * 4 + sizeof(Segment) * 2;
+    const size_t per_scene = sizeof(StepParams) + 4 + sphk::kBodiesPerScene * 4 + sizeof(Segment) * 2 + 4 * 8;
     return slots * per_slot + cells * per_cell + tiles * 4 + scenes * per_scene;
 }
 
@@ -278,7 +287,49 @@ void release(sph_batch *b) {
 }
 
 sphk::BatchTable table(const sph_batch *b) {
-    return sphk::BatchTable{b->d_scenes, b->scene_of, b->d_steps_left, b->used_slots};
+    return sphk::BatchTable{b->d_scenes, b->scene_of, b->d_steps_left, b->used_slots, nullptr, nullptr, 0};
+}
+
+enum MapIndex { kMapFluid = 0, kMapCorot = 1, kMapSvk = 2, kMapSolids = 3 };
+
+// The scene table restricted to the chunks of one system's running scenes.
+sphk::BatchTable table(const sph_batch *b, int map) {
+    sphk::BatchTable t = table(b);
+    t.chunk_slot = b->d_chunk_slot[map];
+    t.chunk_end = b->d_chunk_end[map];
+    t.nchunks = b->nchunks[map];
+    return t;
+}
+
+// Chunk maps of the scenes with steps left, in slot order.
+int build_chunk_maps(sph_batch *b) {
+    if (b->maps_version == b->layout_version) return 0;
+    std::vector<unsigned> slot[sph_batch::kMaps], end[sph_batch::kMaps];
+    for (const Segment &g : b->segs) {
+        if (g.scene == kNoScene || b->steps_left[g.scene] == 0) continue;
+        const StepParams &p = b->scenes[g.scene];
+        const int m = p.system == sphk::kFluid ? kMapFluid : p.system == sphk::kCorotational ? kMapCorot : kMapSvk;
+        for (unsigned c = 0; c < p.n; c += sphk::kChunk) {
+            slot[m].push_back(p.p0 + c);
+            end[m].push_back(p.p0 + p.n);
+            if (m != kMapFluid) {
+                slot[kMapSolids].push_back(p.p0 + c);
+                end[kMapSolids].push_back(p.p0 + p.n);
+            }
+        }
+    }
+    for (int m = 0; m < sph_batch::kMaps; ++m) {
+        if (slot[m].size() > b->max_chunks) return fail("internal: chunk map overflow");
+        b->nchunks[m] = static_cast<unsigned>(slot[m].size());
+        if (slot[m].empty()) continue;
+        SPH_HIP_TRY(hipMemcpyAsync(b->d_chunk_slot[m], slot[m].data(), slot[m].size() * 4, hipMemcpyHostToDevice,
+                                   b->stream));
+        SPH_HIP_TRY(hipMemcpyAsync(b->d_chunk_end[m], end[m].data(), end[m].size() * 4, hipMemcpyHostToDevice,
+                                   b->stream));
+    }
+    SPH_HIP_TRY(hipStreamSynchronize(b->stream));   // the host vectors go out of scope
+    b->maps_version = b->layout_version;
+    return 0;
 }
 
 sphk::FieldPtrs field_ptrs(const sph_batch *b) {
@@ -423,6 +474,8 @@ int enqueue_bins(sph_batch *b) {
 int enqueue_step(sph_batch *b, unsigned systems, bool setup) {
     hipStream_t s = b->stream;
     const sphk::BatchTable t = table(b);
+    const sphk::BatchTable tf = table(b, kMapFluid), tc = table(b, kMapCorot), tv = table(b, kMapSvk);
+    const sphk::BatchTable ts = table(b, kMapSolids);
     const int kahan = b->compensated, fp64 = b->polar_fp64;
     const unsigned solid_mask = (1u << sphk::kCorotational) | (1u << sphk::kSvk);
     const sphk::SceneFilter fluid{1u << sphk::kFluid, 0u}, solids{solid_mask, 0u};
@@ -439,27 +492,30 @@ int enqueue_step(sph_batch *b, unsigned systems, bool setup) {
         return -1;
     if ((systems & (1u << sphk::kFluid)) &&
         (timed(b, "compute_density_fluid",
-               [&] { return sphk::launch_batch_density_listed(t, fluid, f, b->pairs, kahan, s); }) ||
+               [&] { return sphk::launch_batch_density_listed(tf, fluid, f, b->pairs, kahan, s); }) ||
          timed(b, "compute_forces_fluids",
-               [&] { return sphk::launch_batch_forces_fluids_listed(t, fluid, f, b->pairs, kahan, s); })))
+               [&] { return sphk::launch_batch_forces_fluids_listed(tf, fluid, f, b->pairs, kahan, s); })))
         return -1;
-    if ((systems & solid_mask) &&
-        timed(b, "compute_density_solid", [&] { return sphk::launch_batch_density(t, solids, f, kahan, s); }))
+    sphk::PairCache solid_pairs = b->pairs;
+    solid_pairs.valid = sphk::kSolidList;
+    if ((systems & solid_mask) && timed(b, "compute_density_solid", [&] {
+            return sphk::launch_batch_density_solid_listed(ts, solids, f, solid_pairs, kahan, s);
+        }))
         return -1;
     if ((systems & (1u << sphk::kCorotational)) &&
         (timed(b, "compute_rotations_and_strains",
-               [&] { return sphk::launch_batch_rotations_and_strains(t, corot, f, kahan, fp64, s); }) ||
+               [&] { return sphk::launch_batch_rotations_and_strains(tc, corot, f, solid_pairs, kahan, fp64, s); }) ||
          timed(b, "compute_stresses_corotational",
-               [&] { return sphk::launch_batch_stresses_corotational(t, corot, f, s); }) ||
+               [&] { return sphk::launch_batch_stresses_corotational(tc, corot, f, s); }) ||
          timed(b, "compute_forces_solids_corotational",
-               [&] { return sphk::launch_batch_forces_solids(t, corot, f, true, kahan, s); })))
+               [&] { return sphk::launch_batch_forces_solids(tc, corot, f, solid_pairs, true, kahan, s); })))
         return -1;
     if ((systems & (1u << sphk::kSvk)) &&
         (timed(b, "compute_deformations",
-               [&] { return sphk::launch_batch_deformations_svk(t, svk, f, kahan, fp64, s); }) ||
-         timed(b, "compute_stresses_svk", [&] { return sphk::launch_batch_stresses_svk(t, svk, f, s); }) ||
+               [&] { return sphk::launch_batch_deformations_svk(tv, svk, f, solid_pairs, kahan, fp64, s); }) ||
+         timed(b, "compute_stresses_svk", [&] { return sphk::launch_batch_stresses_svk(tv, svk, f, s); }) ||
          timed(b, "compute_forces_solids_svk",
-               [&] { return sphk::launch_batch_forces_solids(t, svk, f, false, kahan, s); })))
+               [&] { return sphk::launch_batch_forces_solids(tv, svk, f, solid_pairs, false, kahan, s); })))
         return -1;
     if (timed(b, "step_forward", [&] { return sphk::launch_batch_step_forward(t, sphk::kAllScenes, f, s); }) ||
         timed(b, "batch_tick", [&] {
@@ -605,6 +661,7 @@ extern "C" int sph_batch_create(unsigned slots, unsigned cells, unsigned scenes,
     if (!out) return fail("sph_batch_create: out is NULL");
     *out = nullptr;
     if (slots == 0 || cells == 0 || scenes == 0) return fail("sph_batch_create: empty pool");
+    if (slots > sphk::kListSlot) return fail("sph_batch_create: at most %u slots", sphk::kListSlot);
     sph_batch *b = new sph_batch();
     if (sphhip::current_device(&b->device)) {
         delete b;
@@ -640,6 +697,9 @@ extern "C" int sph_batch_create(unsigned slots, unsigned cells, unsigned scenes,
          dev_alloc(b, &b->bins.tile_sums, tiles) || dev_alloc(b, &b->d_scenes, scenes) ||
          dev_alloc(b, &b->d_steps_left, scenes) || dev_alloc(b, &b->body_dynamic, sphk::kBodiesPerScene * scenes) ||
          dev_alloc(b, &b->d_segs, 2 * static_cast<size_t>(scenes) + 1);
+    b->max_chunks = S / sphk::kChunk + scenes;
+    for (int m = 0; m < sph_batch::kMaps && !rc; ++m)
+        rc = dev_alloc(b, &b->d_chunk_slot[m], b->max_chunks) || dev_alloc(b, &b->d_chunk_end[m], b->max_chunks);
     if (rc) {
         release(b);
         return -1;
@@ -787,7 +847,7 @@ extern "C" int sph_batch_set_graph_steps(sph_batch *b, unsigned steps_per_graph)
 extern "C" int sph_batch_step(sph_batch *b, unsigned num_steps) {
     if (!b) return fail("null batch");
     if (hipSetDevice(b->device) != hipSuccess) return fail("hipSetDevice failed");
-    if (upload_layout(b)) return -1;
+    if (upload_layout(b) || build_chunk_maps(b)) return -1;
     unsigned done = 0;
     while (done < num_steps) {
         if (b->used_slots == 0) break;

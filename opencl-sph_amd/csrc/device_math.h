@@ -77,6 +77,20 @@ __device__ __forceinline__ V3 grad_spiky(V3 v, const KernelConsts &k) {
     float s = (k.h - r) * (k.h - r);
     return (dir * k.spiky) * s;
 }
+// grad_spiky(v) and grad_spiky(-v) together.  Negation is exact and every
+// operation of grad_spiky is sign-symmetric under round-to-nearest, so
+// grad_spiky(-v) is bitwise -grad_spiky(v) when |v| <= h (and +0 otherwise,
+// as grad_spiky returns).
+__device__ __forceinline__ void grad_spiky_pm(V3 v, const KernelConsts &k, V3 *plus, V3 *minus) {
+    const float r = length(v);
+    if (r > k.h) {
+        *plus = *minus = V3{0.0f, 0.0f, 0.0f};
+        return;
+    }
+    const V3 g = ((v / r) * k.spiky) * ((k.h - r) * (k.h - r));
+    *plus = g;
+    *minus = -g;
+}
 // lap W_visc(r)   (kernels/common.cl:88-90)
 __device__ __forceinline__ float lap_visc(float r, const KernelConsts &k) {
     if (r > k.h) return 0.0f;

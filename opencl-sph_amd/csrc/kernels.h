@@ -111,28 +111,47 @@ hipError_t launch_scatter_words(const void *src, void *dst, const unsigned *map,
                                 unsigned count, unsigned words, hipStream_t s);
 
 // ---- per-particle passes (kernels_fluid.hip, kernels_solid.hip) ------------
-// In-range neighbour lists written by the density pass: for slot i, entries
-// k < count[i] at list[k * stride + i] are the slots j with length(x_j - x_i) <= h
-// in the reference's visiting order (self included).  count[i] > kListCap
-// means the list overflowed and readers must scan the candidates instead.
+// Neighbour lists written by the density pass, entries k < count[i] at
+// list[k * stride + i] in the reference's visiting order (self included);
+// count[i] > kListCap means the list overflowed (or was not built for slot
+// i) and readers scan the candidates instead.
+//   kFluidList: the slots j with length(x_j - x_i) <= h.
+//   kSolidList: the slots j with length(x_j - x_i) <= h (flag
+//     kListCurrent) or length(x0_j - x0_i) <= h (flag kListReference), the
+//     union of the current and reference neighbourhoods that the solid passes
+//     iterate; slot = entry & kListSlot.
 constexpr unsigned kListCap = 48;
+constexpr int kFluidList = 1;
+constexpr int kSolidList = 2;
+constexpr unsigned kListCurrent = 1u << 31;
+constexpr unsigned kListReference = 1u << 30;
+constexpr unsigned kListSlot = kListReference - 1u;   // solid lists need slots < 2^30
 struct PairCache {
     unsigned *list;        // [kListCap * stride]
     unsigned *count;       // [stride]
-    float *pressure;       // [stride] equation of state of each slot's density
+    float *pressure;       // [stride] equation of state of each slot's density (fluid lists)
     unsigned stride;       // slots per list column (pnum, or the batch capacity)
-    int valid;             // written by the density pass of the current state
+    int valid;             // kFluidList / kSolidList: built from the current state; 0: stale
 };
 
 // Per-scene parameters of a batch: slot s belongs to scene scene_of[s]
 // (kNoScene for unused slots) and is active when s < p0 + n of that scene
 // and the scene has steps left.
 constexpr unsigned kNoScene = 0xffffffffu;
+// A launch covers either the slots [0, capacity), or, with a chunk map, the
+// 64-slot chunks listed in it: chunk k holds slots [chunk_slot[k],
+// min(chunk_slot[k] + 64, chunk_end[k])) of one scene (chunk_end = p0 + n),
+// so each wavefront works on one scene and a pass for one system launches
+// only over that system's scenes, spread evenly over the XCDs.
+constexpr unsigned kChunk = 64;
 struct BatchTable {
     const StepParams *scenes;
     const unsigned *scene_of;
     const unsigned *steps_left;   // per scene
-    unsigned capacity;            // slots a launch covers (the pool's used slots)
+    unsigned capacity;            // the pool's used slots
+    const unsigned *chunk_slot;   // nullptr: no chunk map
+    const unsigned *chunk_end;
+    unsigned nchunks;
 };
 
 hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipStream_t s);
@@ -141,16 +160,21 @@ hipError_t launch_compute_density_listed(const StepParams &p, const FieldPtrs &f
                                          hipStream_t s);
 hipError_t launch_forces_fluids_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
                                        hipStream_t s);
+// Density that also fills a kSolidList pair cache (solid programs).
+hipError_t launch_compute_density_solid_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
+                                               hipStream_t s);
 hipError_t launch_compute_original_density(const StepParams &p, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_forces_fluids(const StepParams &p, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_step_forward(const StepParams &p, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_plane_constraints(const StepParams &p, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_init_original_position(const StepParams &p, const FieldPtrs &f, hipStream_t s);
-hipError_t launch_rotations_and_strains(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+// The solid neighbour passes iterate `c` when c.valid == kSolidList.
+hipError_t launch_rotations_and_strains(const StepParams &p, const FieldPtrs &f, const PairCache &c,
+                                        hipStream_t s);
 hipError_t launch_stresses_corotational(const StepParams &p, const FieldPtrs &f, hipStream_t s);
-hipError_t launch_deformations_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s);
+hipError_t launch_deformations_svk(const StepParams &p, const FieldPtrs &f, const PairCache &c, hipStream_t s);
 hipError_t launch_stresses_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s);
-hipError_t launch_forces_solids(const StepParams &p, const FieldPtrs &f, bool corotational,
+hipError_t launch_forces_solids(const StepParams &p, const FieldPtrs &f, const PairCache &c, bool corotational,
                                 hipStream_t s);
 
 // ---- batches: the same kernels with per-scene parameters ------------------
@@ -168,6 +192,8 @@ hipError_t launch_batch_density_listed(const BatchTable &b, SceneFilter sel, con
                                        const PairCache &c, int compensated, hipStream_t s);
 hipError_t launch_batch_density(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, int compensated,
                                 hipStream_t s);
+hipError_t launch_batch_density_solid_listed(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                             const PairCache &c, int compensated, hipStream_t s);
 hipError_t launch_batch_original_density(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
                                          int compensated, hipStream_t s);
 hipError_t launch_batch_init_original_position(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
@@ -176,14 +202,14 @@ hipError_t launch_batch_forces_fluids_listed(const BatchTable &b, SceneFilter se
                                              const PairCache &c, int compensated, hipStream_t s);
 hipError_t launch_batch_step_forward(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_batch_rotations_and_strains(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
-                                              int compensated, int polar_fp64, hipStream_t s);
+                                              const PairCache &c, int compensated, int polar_fp64, hipStream_t s);
 hipError_t launch_batch_stresses_corotational(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
                                               hipStream_t s);
 hipError_t launch_batch_deformations_svk(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
-                                         int compensated, int polar_fp64, hipStream_t s);
+                                         const PairCache &c, int compensated, int polar_fp64, hipStream_t s);
 hipError_t launch_batch_stresses_svk(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, hipStream_t s);
 hipError_t launch_batch_forces_solids(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
-                                      bool corotational, int compensated, hipStream_t s);
+                                      const PairCache &c, bool corotational, int compensated, hipStream_t s);
 
 // FP64 reductions over slots [0, n): momentum (3), kinetic energy, centre of
 // mass (3).  `partials` holds 7 * kReduceBlocks doubles.

@@ -39,8 +39,6 @@ struct Acc<true> {
     __device__ __forceinline__ float value() const { return s; }
 };
 
-__device__ __forceinline__ unsigned linear_slot() { return blockIdx.x * blockDim.x + threadIdx.x; }
-
 // Equation of state: the reference's linear p = k (rho - rho0)
 // (kernels/fluids.cl:28-29), or Tait with exponent 7 when the conf sets
 // eos_tait.
@@ -94,7 +92,7 @@ __device__ __forceinline__ float density_scan(const StepParams &p, const FieldPt
 // ORIGINAL = compute_original_density (density0, no neighbour count).
 template <class PS, bool ORIGINAL, bool KAHAN>
 __global__ void __launch_bounds__(kBlock) k_density(PS ps, FieldPtrs f) {
-    const unsigned i = xcd_slot(kBlock);
+    const unsigned i = ps.slot_xcd(kBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const V3 xi = load3(f.position, i);
@@ -143,7 +141,7 @@ __device__ __forceinline__ bool row_range(const GridGeom &g, unsigned lo, unsign
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs f, PairCache c) {
     __shared__ unsigned short buf[kListCap * kListBlock];
-    const unsigned i = xcd_slot(kListBlock);
+    const unsigned i = ps.slot_xcd(kListBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
@@ -209,6 +207,112 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     c.pressure[i] = pressure(p, out);
 }
 
+// compute_density for solid programs, which also records the kSolidList
+// neighbour list (current or reference distance within h, with flags) in
+// candidate order.  Two phases per lane as in k_density_listed: the scan
+// compacts candidates in either range into LDS as 16-bit entries (row << 12
+// | current << 11 | reference << 10 | offset in row), then the kernel sum
+// runs over the compacted entries.  The range tests are the
+// square-root-free form of length(d) <= h.  Lanes near a mirror plane take
+// the reference loop and mark their list as overflowed; lanes with more than
+// kListCap entries or a row longer than 1024 slots scan again without LDS.
+constexpr unsigned kSolidOffsetBits = 10;
+
+template <class PS, bool KAHAN>
+__global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, FieldPtrs f, PairCache c) {
+    __shared__ unsigned short buf[kListCap * kListBlock];
+    const unsigned i = ps.slot_xcd(kListBlock);
+    if (!ps.active(i)) return;
+    const StepParams &p = ps.at(i);
+    const KernelConsts kc = kconsts(p);
+    const float *__restrict__ pos = f.position;
+    const float *__restrict__ opos = f.originalpos;
+    const unsigned lo = p.p0, hi = p.p0 + p.n;
+    const float r2_out = p.r2_out;
+    const GridGeom g = p.grid;
+    const V3 xi = load3(pos, i);
+    const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
+    const unsigned key = f.gridcell[i];
+    unsigned short *col = buf + threadIdx.x;
+    unsigned nn = 0, cnt = 0;
+    float rho;
+    if (inear) {
+        rho = density_scan<KAHAN, true>(p, f, i, xi, inear, &nn, nullptr, 0, nullptr);
+        cnt = kListCap + 1u;
+    } else {
+        const V3 x0i = load3(opos, i);
+        bool fast = key >= g.c0 && key - g.c0 < g.ncells;
+        unsigned xo = 0, yo = 0, zo = 0;
+        if (fast) {
+            const unsigned local = key - g.c0;
+            zo = local / g.layer;
+            const unsigned rem = local - g.layer * zo;
+            yo = rem / g.rx;
+            xo = rem - g.rx * yo;
+#pragma unroll 1
+            for (unsigned r = 0; r < 9u; ++r) {
+                unsigned b, e;
+                if (!row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e)) continue;
+                if (e - b > (1u << kSolidOffsetBits)) fast = false;
+                for (unsigned j = b; j < e; ++j) {
+                    const bool cur = !(sqlength(load3(pos, j) - xi) >= r2_out);
+                    const bool ref = !(sqlength(load3(opos, j) - x0i) >= r2_out);
+                    if (cur || ref) {
+                        if (cnt < kListCap)
+                            col[cnt * kListBlock] = static_cast<unsigned short>(
+                                (r << kRowBits) | (cur ? 1u << 11 : 0u) | (ref ? 1u << 10 : 0u) | (j - b));
+                        ++cnt;
+                    }
+                }
+            }
+            if (cnt > kListCap) fast = false;
+        }
+        Acc<KAHAN> acc;
+        if (fast) {
+            unsigned cur_row = 0xffu, base = 0, e = 0;
+            for (unsigned k = 0; k < cnt; ++k) {
+                const unsigned ent = col[k * kListBlock];
+                const unsigned r = ent >> kRowBits;
+                if (r != cur_row) {
+                    cur_row = r;
+                    row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &base, &e);
+                }
+                const unsigned j = base + (ent & ((1u << kSolidOffsetBits) - 1u));
+                const bool cur = (ent >> 11) & 1u;
+                if (cur) {
+                    const float rr = length(load3(pos, j) - xi);
+                    acc.add(w_poly6(rr, kc));
+                    if (rr < kc.h) ++nn;
+                }
+                c.list[static_cast<size_t>(k) * c.stride + i] =
+                    j | (cur ? kListCurrent : 0u) | (((ent >> 10) & 1u) ? kListReference : 0u);
+            }
+        } else {
+            cnt = 0;
+            for_each_candidate(g, lo, hi, key, f.celloffset, f.gridcount, [&](unsigned j) {
+                const V3 d = load3(pos, j) - xi;
+                const bool cur = !(sqlength(d) >= r2_out);
+                const bool ref = !(sqlength(load3(opos, j) - x0i) >= r2_out);
+                if (cur) {
+                    const float rr = length(d);
+                    acc.add(w_poly6(rr, kc));
+                    if (rr < kc.h) ++nn;
+                }
+                if (cur || ref) {
+                    if (cnt < kListCap)
+                        c.list[static_cast<size_t>(cnt) * c.stride + i] =
+                            j | (cur ? kListCurrent : 0u) | (ref ? kListReference : 0u);
+                    ++cnt;
+                }
+            });
+        }
+        rho = acc.value();
+    }
+    f.density[i] = rho * p.mass;
+    if (f.num_neighbours) f.num_neighbours[i] = nn;
+    c.count[i] = cnt;
+}
+
 // Penalty walls at +-2 m, evaluated in double as in the reference macro
 // (kernels/common.cl:37-43 with the double literals of kernels/fluids.cl:60).
 __device__ __forceinline__ float cube_wall(float x, float fc) {
@@ -227,14 +331,14 @@ __device__ __forceinline__ float cube_wall(float x, float fc) {
 template <bool KAHAN>
 struct FluidForce {
     Acc<KAHAN> fx, fy, fz;
-    __device__ __forceinline__ void pair(const StepParams &p, const KernelConsts &kc, V3 xi, V3 vi, float rhoi,
+    __device__ __forceinline__ void pair(float viscosity, const KernelConsts &kc, V3 xi, V3 vi, float rhoi,
                                          float pi, V3 xj, V3 vj, float rhoj, float pj) {
         const V3 d = xj - xi;
         const float r = length(d);
         const V3 rv = vj - vi;
         const float din = kc.h - r;
         const float didj = rhoi * rhoj;
-        const float vterm = (p.viscosity * din) * kc.lap;
+        const float vterm = (viscosity * din) * kc.lap;
         // (p_i + p_j) * 0.5 / r * din * din * spiky, in double
         // (the 0.5 literal promotes it, kernels/fluids.cl:48)
         const float pterm = static_cast<float>(static_cast<double>(pi + pj) * 0.5 / static_cast<double>(r) *
@@ -256,7 +360,7 @@ struct FluidForce {
 
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
-    const unsigned i = xcd_slot(kBlock);
+    const unsigned i = ps.slot_xcd(kBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
@@ -266,13 +370,14 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
     const V3 xi = load3(pos, i), vi = load3(vel, i);
     const float rhoi = rho[i];
     const float pi = pressure(p, rhoi);
+    const float visc = p.viscosity;   // read once: with a scene table p is in global memory
     FluidForce<KAHAN> acc;
     for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         if (j == i) return;
         const V3 xj = load3(pos, j);
         if (length(xj - xi) > kc.h) return;
         const float rhoj = rho[j];
-        acc.pair(p, kc, xi, vi, rhoi, pi, xj, load3(vel, j), rhoj, pressure(p, rhoj));
+        acc.pair(visc, kc, xi, vi, rhoi, pi, xj, load3(vel, j), rhoj, pressure(p, rhoj));
     });
     store3(f.force, i, acc.finish(p, xi));
 }
@@ -282,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
 // equation of state per pair).  Overflowed lists fall back to the scan.
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtrs f, PairCache c) {
-    const unsigned i = xcd_slot(kBlock);
+    const unsigned i = ps.slot_xcd(kBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
@@ -293,6 +398,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
     const V3 xi = load3(pos, i), vi = load3(vel, i);
     const float rhoi = rho[i];
     const float pi = prs[i];
+    const float visc = p.viscosity;   // read once: with a scene table p is in global memory
     FluidForce<KAHAN> acc;
     const unsigned cnt = c.count[i];
     if (cnt <= kListCap) {
@@ -301,7 +407,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
         unsigned j = cnt ? c.list[i] : 0u;
         for (unsigned k = 0; k < cnt; ++k) {
             const unsigned jn = k + 1 < cnt ? c.list[static_cast<size_t>(k + 1) * c.stride + i] : 0u;
-            if (j != i) acc.pair(p, kc, xi, vi, rhoi, pi, load3(pos, j), load3(vel, j), rho[j], prs[j]);
+            if (j != i) acc.pair(visc, kc, xi, vi, rhoi, pi, load3(pos, j), load3(vel, j), rho[j], prs[j]);
             j = jn;
         }
     } else {
@@ -309,7 +415,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
             if (j == i) return;
             const V3 xj = load3(pos, j);
             if (length(xj - xi) > kc.h) return;
-            acc.pair(p, kc, xi, vi, rhoi, pi, xj, load3(vel, j), rho[j], prs[j]);
+            acc.pair(visc, kc, xi, vi, rhoi, pi, xj, load3(vel, j), rho[j], prs[j]);
         });
     }
     store3(f.force, i, acc.finish(p, xi));
@@ -324,7 +430,7 @@ __device__ __forceinline__ bool is_static_body(const StepParams &p, const FieldP
 
 template <class PS>
 __global__ void __launch_bounds__(kBlock) k_step_forward(PS ps, FieldPtrs f) {
-    const unsigned i = linear_slot();
+    const unsigned i = ps.slot_linear(kBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     if (is_static_body(p, f, i)) return;
@@ -340,7 +446,7 @@ __global__ void __launch_bounds__(kBlock) k_step_forward(PS ps, FieldPtrs f) {
 
 template <class PS>
 __global__ void __launch_bounds__(kBlock) k_plane_constraints(PS ps, FieldPtrs f) {
-    const unsigned i = linear_slot();
+    const unsigned i = ps.slot_linear(kBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const unsigned mask = f.plane_constraints_particles[i];
@@ -359,7 +465,7 @@ __global__ void __launch_bounds__(kBlock) k_plane_constraints(PS ps, FieldPtrs f
 
 template <class PS>
 __global__ void __launch_bounds__(kBlock) k_init_original_position(PS ps, FieldPtrs f) {
-    const unsigned i = linear_slot();
+    const unsigned i = ps.slot_linear(kBlock);
     if (!ps.active(i)) return;
     store3(f.originalpos, i, load3(f.position, i));
 }
@@ -434,6 +540,16 @@ hipError_t density_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, 
 }
 
 template <class PS>
+hipError_t density_solid_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated,
+                                hipStream_t s) {
+    if (ps.slots() == 0) return hipSuccess;
+    const dim3 g = nbr_grid(ps.slots(), kListBlock), b(kListBlock);
+    if (compensated & kKahanDensity) hipLaunchKernelGGL((k_density_solid_listed<PS, true>), g, b, 0, s, ps, f, c);
+    else hipLaunchKernelGGL((k_density_solid_listed<PS, false>), g, b, 0, s, ps, f, c);
+    return hipGetLastError();
+}
+
+template <class PS>
 hipError_t forces_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated, hipStream_t s) {
     if (ps.slots() == 0) return hipSuccess;
     const dim3 g = nbr_grid(ps.slots(), kBlock), b(kBlock);
@@ -470,6 +586,11 @@ hipError_t launch_compute_density_listed(const StepParams &p, const FieldPtrs &f
 hipError_t launch_forces_fluids_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
                                        hipStream_t s) {
     return forces_listed(SingleScene{p}, f, c, p.compensated, s);
+}
+
+hipError_t launch_compute_density_solid_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
+                                               hipStream_t s) {
+    return density_solid_listed(SingleScene{p}, f, c, p.compensated, s);
 }
 
 hipError_t launch_compute_original_density(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
@@ -515,6 +636,11 @@ hipError_t launch_batch_density_listed(const BatchTable &b, SceneFilter sel, con
 hipError_t launch_batch_density(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, int compensated,
                                 hipStream_t s) {
     return density(SceneTable{b, sel}, f, false, compensated, s);
+}
+
+hipError_t launch_batch_density_solid_listed(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
+                                             const PairCache &c, int compensated, hipStream_t s) {
+    return density_solid_listed(SceneTable{b, sel}, f, c, compensated, s);
 }
 
 hipError_t launch_batch_original_density(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,

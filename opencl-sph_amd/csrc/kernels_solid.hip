@@ -106,14 +106,37 @@ __device__ __forceinline__ void deformation_from(const float *apq, const float *
     for (int k = 0; k < 9; ++k) def[k] = static_cast<float>(out[k]);
 }
 
+// Which lanes a launch of a solid neighbour pass handles.  With a kSolidList
+// cache the pass runs as two kernels, one over the lanes whose list is
+// complete and one scanning the candidates of the others (overflow, mirror
+// planes: the density pass marks those lanes as overflowed), so that each
+// kernel has only its own loop's register needs.  Without a cache one kernel
+// scans every lane.
+enum ListMode { kScanAll = 0, kListedOnly = 1, kScanOverflow = 2 };
+
+// Entries of slot i's kSolidList list (> kListCap: scan instead).
+template <int MODE>
+__device__ __forceinline__ unsigned solid_list_count(const PairCache &c, unsigned i) {
+    return MODE == kScanAll ? kListCap + 1u : c.count[i];
+}
+template <int MODE>
+__device__ __forceinline__ bool skip_lane(unsigned cnt) {
+    return (MODE == kListedOnly && cnt > kListCap) || (MODE == kScanOverflow && cnt <= kListCap);
+}
+__device__ __forceinline__ unsigned list_entry(const PairCache &c, unsigned k, unsigned i) {
+    return c.list[static_cast<size_t>(k) * c.stride + i];
+}
+
 __device__ __forceinline__ bool in_body(const FieldPtrs &f, unsigned i) {
     return f.body_number != nullptr && f.body_number[i] > 0u;
 }
 
-template <class PS, bool KAHAN, bool FP64>
-__global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f) {
-    const unsigned i = xcd_slot(kBlock);
+template <class PS, bool KAHAN, bool FP64, int MODE>
+__global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f, PairCache c) {
+    const unsigned i = ps.slot_xcd(kBlock);
     if (!ps.active(i) || in_body(f, i)) return;
+    const unsigned cnt = solid_list_count<MODE>(c, i);
+    if (skip_lane<MODE>(cnt)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
@@ -124,7 +147,17 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
     const unsigned key = f.gridcell[i];
 
     Acc<KAHAN> apq[9];
-    for_each_candidate(p.grid, p.p0, p.p0 + p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
+    if (MODE == kListedOnly) {
+        unsigned next = cnt ? list_entry(c, 0, i) : 0u;
+        for (unsigned k = 0; k < cnt; ++k) {
+            const unsigned e = next;
+            const unsigned j = e & kListSlot;
+            next = k + 1 < cnt ? list_entry(c, k + 1, i) : 0u;
+            if (!(e & kListCurrent) || j == i) continue;
+            const V3 pv = load3(pos, j) - xi;
+            add_apq(apq, pv, load3(opos, j) - x0i, w_poly6(length(pv), kc));
+        }
+    } else for_each_candidate(p.grid, p.p0, p.p0 + p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 pv = xj - xi;
@@ -156,14 +189,26 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
     const float rt[9] = {rot[0], rot[3], rot[6], rot[1], rot[4], rot[7], rot[2], rot[5], rot[8]};
 
     const float d0i = f.density0[i];
+    const float mass = p.mass;   // read once (see k_forces_solids)
     Acc<KAHAN> du[9];
-    for_each_candidate(p.grid, p.p0, p.p0 + p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
+    if (MODE == kListedOnly) {
+        unsigned next = cnt ? list_entry(c, 0, i) : 0u;
+        for (unsigned k = 0; k < cnt; ++k) {
+            const unsigned e = next;
+            const unsigned j = e & kListSlot;
+            next = k + 1 < cnt ? list_entry(c, k + 1, i) : 0u;
+            if (!(e & kListReference) || j == i) continue;
+            const V3 q = load3(opos, j) - x0i;
+            const V3 u = mat_mul(rt, load3(pos, j) - xi) - q;
+            add_outer(du, (u * mass) / d0i, grad_spiky(-q, kc));
+        }
+    } else for_each_candidate(p.grid, p.p0, p.p0 + p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 q = x0j - x0i;
             if (!(length(q) > kc.h)) {
                 const V3 u = mat_mul(rt, xj - xi) - q;
-                add_outer(du, (u * p.mass) / d0i, grad_spiky(-q, kc));
+                add_outer(du, (u * mass) / d0i, grad_spiky(-q, kc));
             }
         }
         if (inear) {
@@ -174,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
                     const V3 pv = mat_mul(rt, reflect_combo(p, b, xj) - xi);
                     const V3 q = reflect_combo(p, b, x0j) - x0i;
                     const V3 u = pv - q;
-                    if (length(pv) < kc.h) add_outer(du, (u * p.mass) / d0i, grad_spiky(-q, kc));
+                    if (length(pv) < kc.h) add_outer(du, (u * mass) / d0i, grad_spiky(-q, kc));
                 }
         }
     });
@@ -192,7 +237,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
 // sigma = C : eps, isotropic   (kernels/solid_corotation.cl:9-26,197-208)
 template <class PS>
 __global__ void __launch_bounds__(kBlock) k_stresses_corot(PS ps, FieldPtrs f) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = ps.slot_linear(kBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const float E = p.youngs, nu = p.poisson;
@@ -210,10 +255,12 @@ __global__ void __launch_bounds__(kBlock) k_stresses_corot(PS ps, FieldPtrs f) {
                c[r][5] * ev[5];
 }
 
-template <class PS, bool KAHAN, bool FP64>
-__global__ void __launch_bounds__(kBlock) k_deformations_svk(PS ps, FieldPtrs f) {
-    const unsigned i = xcd_slot(kBlock);
+template <class PS, bool KAHAN, bool FP64, int MODE>
+__global__ void __launch_bounds__(kBlock) k_deformations_svk(PS ps, FieldPtrs f, PairCache c) {
+    const unsigned i = ps.slot_xcd(kBlock);
     if (!ps.active(i)) return;
+    const unsigned cnt = solid_list_count<MODE>(c, i);
+    if (skip_lane<MODE>(cnt)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
@@ -222,7 +269,20 @@ __global__ void __launch_bounds__(kBlock) k_deformations_svk(PS ps, FieldPtrs f)
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned ncombo = 1u << p.nplanes;
     Acc<KAHAN> apq[9], aqq[6];
-    for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+    if (MODE == kListedOnly) {
+        unsigned next = cnt ? list_entry(c, 0, i) : 0u;
+        for (unsigned k = 0; k < cnt; ++k) {
+            const unsigned e = next;
+            const unsigned j = e & kListSlot;
+            next = k + 1 < cnt ? list_entry(c, k + 1, i) : 0u;
+            if (!(e & kListCurrent) || j == i) continue;
+            const V3 pv = load3(pos, j) - xi;
+            const V3 q = load3(opos, j) - x0i;
+            const float w = w_poly6(length(pv), kc);
+            add_apq(apq, pv, q, w);
+            add_aqq(aqq, q, w);
+        }
+    } else for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 pv = xj - xi;
@@ -262,7 +322,7 @@ __global__ void __launch_bounds__(kBlock) k_deformations_svk(PS ps, FieldPtrs f)
 // literals of kernels/solid_svk.cl:1-43 kept as double arithmetic.
 template <class PS>
 __global__ void __launch_bounds__(kBlock) k_stresses_svk(PS ps, FieldPtrs f) {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = ps.slot_linear(kBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     float F[9], Ft[9], ftf[9];
@@ -301,10 +361,12 @@ __global__ void __launch_bounds__(kBlock) k_stresses_svk(PS ps, FieldPtrs f) {
     s[S12] = sc * cauchy[5];
 }
 
-template <class PS, bool CORO, bool KAHAN>
-__global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f) {
-    const unsigned i = xcd_slot(kBlock);
+template <class PS, bool CORO, bool KAHAN, int MODE>
+__global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, PairCache c) {
+    const unsigned i = ps.slot_xcd(kBlock);
     if (!ps.active(i)) return;
+    const unsigned cnt = solid_list_count<MODE>(c, i);
+    if (skip_lane<MODE>(cnt)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
@@ -320,8 +382,40 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f) {
     const float d0i = f.density0[i];
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned ncombo = 1u << p.nplanes;
+    // Scene scalars read once: with a scene table `p` lives in global memory
+    // and the pair loops would otherwise reload it per pair.
+    const float mass = p.mass;
     Acc3<KAHAN> fe, fv, fec, fvc;
-    for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
+    // One in-range pair, direct term (kernels/solid_corotation.cl:262-300).
+    auto pair = [&](unsigned j, V3 xj, V3 x0j, V3 vj, float vivj, const float *sj, const float *rj, float mrho) {
+        const V3 X = x0i - x0j;
+        const float rcur = length(xj - xi);
+        V3 g, gm;   // grad_spiky(X), grad_spiky(-X)
+        grad_spiky_pm(X, kc, &g, &gm);
+        const V3 fji = -vivj * sym_mul(si, g);
+        const V3 fij = (f.body_number && f.body_number[j] > 0u) ? -fji : -vivj * sym_mul(sj, gm);
+        const V3 e = CORO ? (-mat_mul(ri, fji)) + mat_mul(rj, fij) : (-fji) + fij;
+        const V3 v = (mrho * (vj - vi)) * lap_visc(rcur, kc);
+        fe.add(e);
+        fv.add(v);
+        fec.add(e);
+        fvc.add(v);
+    };
+    if (MODE == kListedOnly) {
+        unsigned next = cnt ? list_entry(c, 0, i) : 0u;
+        for (unsigned k = 0; k < cnt; ++k) {
+            const unsigned j = next & kListSlot;
+            next = k + 1 < cnt ? list_entry(c, k + 1, i) : 0u;
+            if (j == i) continue;
+            const float vivj = mass * mass / d0i / f.density0[j];
+            float sj[6], rj[9];
+            for (int m = 0; m < 6; ++m) sj[m] = stress[6 * j + m];
+            if (CORO)
+                for (int m = 0; m < 9; ++m) rj[m] = rot[9 * j + m];
+            const float mrho = mass / f.density[j];
+            pair(j, load3(pos, j), load3(opos, j), load3(vel, j), vivj, sj, rj, mrho);
+        }
+    } else for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 x0j = load3(opos, j);
         const V3 xj = load3(pos, j);
         const V3 X = x0i - x0j;
@@ -330,23 +424,13 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f) {
         const bool near_pair = !(rx > kc.h) || !(rcur > kc.h);
         if (!near_pair && !inear) return;
         const V3 vj = load3(vel, j);
-        const float vivj = p.mass * p.mass / d0i / f.density0[j];
+        const float vivj = mass * mass / d0i / f.density0[j];
         float sj[6], rj[9];
         for (int k = 0; k < 6; ++k) sj[k] = stress[6 * j + k];
         if (CORO)
             for (int k = 0; k < 9; ++k) rj[k] = rot[9 * j + k];
-        const float mrho = p.mass / f.density[j];
-        if (j != i && near_pair) {
-            const V3 fji = -vivj * sym_mul(si, grad_spiky(X, kc));
-            const V3 fij = (f.body_number && f.body_number[j] > 0u) ? -fji
-                                                                    : -vivj * sym_mul(sj, grad_spiky(-X, kc));
-            const V3 e = CORO ? (-mat_mul(ri, fji)) + mat_mul(rj, fij) : (-fji) + fij;
-            const V3 v = (mrho * (vj - vi)) * lap_visc(rcur, kc);
-            fe.add(e);
-            fv.add(v);
-            fec.add(e);
-            fvc.add(v);
-        }
+        const float mrho = mass / f.density[j];
+        if (j != i && near_pair) pair(j, xj, x0j, vj, vivj, sj, rj, mrho);
         if (inear) {
             const unsigned ok = planes_near_j(p, xj, inear);
             if (ok)
@@ -389,19 +473,54 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f) {
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 inline dim3 nbr_grid(unsigned n) { return dim3(xcd_grid((n + kBlock - 1) / kBlock)); }
 
-template <class PS>
-hipError_t rotations_and_strains(const PS &ps, const FieldPtrs &f, int compensated, int polar_fp64,
-                                 hipStream_t s) {
+// Launches kernel template K<MODE> as the list-only and overflow-scan pair
+// when `c` holds a solid list, else as one scan over every lane.
+template <template <int> class K, class PS>
+hipError_t launch_modes(const PS &ps, const FieldPtrs &f, const PairCache &c, hipStream_t s) {
     if (ps.slots() == 0) return hipSuccess;
     const dim3 g = nbr_grid(ps.slots()), b(kBlock);
-    if (compensated & kKahanForces) {
-        if (polar_fp64) hipLaunchKernelGGL((k_rotations_strains<PS, true, true>), g, b, 0, s, ps, f);
-        else hipLaunchKernelGGL((k_rotations_strains<PS, true, false>), g, b, 0, s, ps, f);
+    if (c.valid == kSolidList) {
+        hipLaunchKernelGGL(K<kListedOnly>::template kernel<PS>, g, b, 0, s, ps, f, c);
+        hipLaunchKernelGGL(K<kScanOverflow>::template kernel<PS>, g, b, 0, s, ps, f, c);
     } else {
-        if (polar_fp64) hipLaunchKernelGGL((k_rotations_strains<PS, false, true>), g, b, 0, s, ps, f);
-        else hipLaunchKernelGGL((k_rotations_strains<PS, false, false>), g, b, 0, s, ps, f);
+        hipLaunchKernelGGL(K<kScanAll>::template kernel<PS>, g, b, 0, s, ps, f, c);
     }
     return hipGetLastError();
+}
+
+template <bool KAHAN, bool FP64>
+struct RotationsFor {
+    template <int MODE>
+    struct M {
+        template <class PS>
+        static constexpr auto kernel = k_rotations_strains<PS, KAHAN, FP64, MODE>;
+    };
+};
+template <bool KAHAN, bool FP64>
+struct DeformationsFor {
+    template <int MODE>
+    struct M {
+        template <class PS>
+        static constexpr auto kernel = k_deformations_svk<PS, KAHAN, FP64, MODE>;
+    };
+};
+template <bool CORO, bool KAHAN>
+struct ForcesFor {
+    template <int MODE>
+    struct M {
+        template <class PS>
+        static constexpr auto kernel = k_forces_solids<PS, CORO, KAHAN, MODE>;
+    };
+};
+
+template <class PS>
+hipError_t rotations_and_strains(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated,
+                                 int polar_fp64, hipStream_t s) {
+    const bool k = (compensated & kKahanForces) != 0;
+    if (k && polar_fp64) return launch_modes<RotationsFor<true, true>::template M>(ps, f, c, s);
+    if (k) return launch_modes<RotationsFor<true, false>::template M>(ps, f, c, s);
+    if (polar_fp64) return launch_modes<RotationsFor<false, true>::template M>(ps, f, c, s);
+    return launch_modes<RotationsFor<false, false>::template M>(ps, f, c, s);
 }
 
 template <class PS>
@@ -412,17 +531,13 @@ hipError_t stresses_corotational(const PS &ps, const FieldPtrs &f, hipStream_t s
 }
 
 template <class PS>
-hipError_t deformations_svk(const PS &ps, const FieldPtrs &f, int compensated, int polar_fp64, hipStream_t s) {
-    if (ps.slots() == 0) return hipSuccess;
-    const dim3 g = nbr_grid(ps.slots()), b(kBlock);
-    if (compensated & kKahanForces) {
-        if (polar_fp64) hipLaunchKernelGGL((k_deformations_svk<PS, true, true>), g, b, 0, s, ps, f);
-        else hipLaunchKernelGGL((k_deformations_svk<PS, true, false>), g, b, 0, s, ps, f);
-    } else {
-        if (polar_fp64) hipLaunchKernelGGL((k_deformations_svk<PS, false, true>), g, b, 0, s, ps, f);
-        else hipLaunchKernelGGL((k_deformations_svk<PS, false, false>), g, b, 0, s, ps, f);
-    }
-    return hipGetLastError();
+hipError_t deformations_svk(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated, int polar_fp64,
+                            hipStream_t s) {
+    const bool k = (compensated & kKahanForces) != 0;
+    if (k && polar_fp64) return launch_modes<DeformationsFor<true, true>::template M>(ps, f, c, s);
+    if (k) return launch_modes<DeformationsFor<true, false>::template M>(ps, f, c, s);
+    if (polar_fp64) return launch_modes<DeformationsFor<false, true>::template M>(ps, f, c, s);
+    return launch_modes<DeformationsFor<false, false>::template M>(ps, f, c, s);
 }
 
 template <class PS>
@@ -433,45 +548,42 @@ hipError_t stresses_svk(const PS &ps, const FieldPtrs &f, hipStream_t s) {
 }
 
 template <class PS>
-hipError_t forces_solids(const PS &ps, const FieldPtrs &f, bool corotational, int compensated, hipStream_t s) {
-    if (ps.slots() == 0) return hipSuccess;
-    const dim3 g = nbr_grid(ps.slots()), b(kBlock);
+hipError_t forces_solids(const PS &ps, const FieldPtrs &f, const PairCache &c, bool corotational, int compensated,
+                         hipStream_t s) {
     const bool k = (compensated & kKahanForces) != 0;
-    if (corotational) {
-        if (k) hipLaunchKernelGGL((k_forces_solids<PS, true, true>), g, b, 0, s, ps, f);
-        else hipLaunchKernelGGL((k_forces_solids<PS, true, false>), g, b, 0, s, ps, f);
-    } else {
-        if (k) hipLaunchKernelGGL((k_forces_solids<PS, false, true>), g, b, 0, s, ps, f);
-        else hipLaunchKernelGGL((k_forces_solids<PS, false, false>), g, b, 0, s, ps, f);
-    }
-    return hipGetLastError();
+    if (corotational && k) return launch_modes<ForcesFor<true, true>::template M>(ps, f, c, s);
+    if (corotational) return launch_modes<ForcesFor<true, false>::template M>(ps, f, c, s);
+    if (k) return launch_modes<ForcesFor<false, true>::template M>(ps, f, c, s);
+    return launch_modes<ForcesFor<false, false>::template M>(ps, f, c, s);
 }
 
 }  // namespace
 
-hipError_t launch_rotations_and_strains(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
-    return rotations_and_strains(SingleScene{p}, f, p.compensated, p.polar_fp64, s);
+hipError_t launch_rotations_and_strains(const StepParams &p, const FieldPtrs &f, const PairCache &c,
+                                        hipStream_t s) {
+    return rotations_and_strains(SingleScene{p}, f, c, p.compensated, p.polar_fp64, s);
 }
 
 hipError_t launch_stresses_corotational(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
     return stresses_corotational(SingleScene{p}, f, s);
 }
 
-hipError_t launch_deformations_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
-    return deformations_svk(SingleScene{p}, f, p.compensated, p.polar_fp64, s);
+hipError_t launch_deformations_svk(const StepParams &p, const FieldPtrs &f, const PairCache &c, hipStream_t s) {
+    return deformations_svk(SingleScene{p}, f, c, p.compensated, p.polar_fp64, s);
 }
 
 hipError_t launch_stresses_svk(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
     return stresses_svk(SingleScene{p}, f, s);
 }
 
-hipError_t launch_forces_solids(const StepParams &p, const FieldPtrs &f, bool corotational, hipStream_t s) {
-    return forces_solids(SingleScene{p}, f, corotational, p.compensated, s);
+hipError_t launch_forces_solids(const StepParams &p, const FieldPtrs &f, const PairCache &c, bool corotational,
+                                hipStream_t s) {
+    return forces_solids(SingleScene{p}, f, c, corotational, p.compensated, s);
 }
 
 hipError_t launch_batch_rotations_and_strains(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
-                                              int compensated, int polar_fp64, hipStream_t s) {
-    return rotations_and_strains(SceneTable{b, sel}, f, compensated, polar_fp64, s);
+                                              const PairCache &c, int compensated, int polar_fp64, hipStream_t s) {
+    return rotations_and_strains(SceneTable{b, sel}, f, c, compensated, polar_fp64, s);
 }
 
 hipError_t launch_batch_stresses_corotational(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
@@ -480,8 +592,8 @@ hipError_t launch_batch_stresses_corotational(const BatchTable &b, SceneFilter s
 }
 
 hipError_t launch_batch_deformations_svk(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
-                                         int compensated, int polar_fp64, hipStream_t s) {
-    return deformations_svk(SceneTable{b, sel}, f, compensated, polar_fp64, s);
+                                         const PairCache &c, int compensated, int polar_fp64, hipStream_t s) {
+    return deformations_svk(SceneTable{b, sel}, f, c, compensated, polar_fp64, s);
 }
 
 hipError_t launch_batch_stresses_svk(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, hipStream_t s) {
@@ -489,8 +601,8 @@ hipError_t launch_batch_stresses_svk(const BatchTable &b, SceneFilter sel, const
 }
 
 hipError_t launch_batch_forces_solids(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
-                                      bool corotational, int compensated, hipStream_t s) {
-    return forces_solids(SceneTable{b, sel}, f, corotational, compensated, s);
+                                      const PairCache &c, bool corotational, int compensated, hipStream_t s) {
+    return forces_solids(SceneTable{b, sel}, f, c, corotational, compensated, s);
 }
 
 }  // namespace sphk

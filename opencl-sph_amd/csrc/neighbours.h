@@ -15,23 +15,6 @@
 
 namespace sphk {
 
-// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
-// b % 8), each XCD with its own L2.  Neighbour passes launch a multiple of 8
-// blocks and turn the hardware block id into a logical one so that XCD x
-// sweeps the contiguous eighth [x * per, (x + 1) * per) of the slots, and the
-// neighbour rows its blocks share stay in its own L2.
-constexpr unsigned kXcds = 8;
-
-__host__ __device__ __forceinline__ unsigned xcd_grid(unsigned blocks) {
-    return (blocks + kXcds - 1) / kXcds * kXcds;
-}
-
-__device__ __forceinline__ unsigned xcd_slot(unsigned block_threads) {
-    const unsigned per = gridDim.x / kXcds;
-    const unsigned logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-    return logical * block_threads + threadIdx.x;
-}
-
 template <typename Body>
 __device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned lo, unsigned hi, unsigned key,
                                                    const unsigned *__restrict__ celloffset,

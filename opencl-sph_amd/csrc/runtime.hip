@@ -344,8 +344,8 @@ KernelSpec spec_for(psdata_hip *h, const std::string &name) {
     const Access partial_if_bodies = has(h, "body_number") ? Access::WriteSome : Access::WriteAll;
     if (name == "compute_density")
         return {"compute_density", true,
-                {{"position", Access::Read, true}, {"density", Access::WriteAll, true},
-                 {"num_neighbours", Access::WriteAll, false}}};
+                {{"position", Access::Read, true}, {"originalpos", Access::Read, false},
+                 {"density", Access::WriteAll, true}, {"num_neighbours", Access::WriteAll, false}}};
     if (name == "compute_forces_fluids")
         return {"compute_forces_fluids", true,
                 {{"position", Access::Read, true}, {"velocity", Access::Read, true},
@@ -435,15 +435,25 @@ int run_kernel(psdata_hip *h, const std::string &internal) {
     sphk::FieldPtrs f = field_ptrs(h);
     hipStream_t s = h->stream;
     int rc = 0;
-    const bool listed = h->pairs.list != nullptr;
+    // Neighbour lists: fluid programs list r <= h and cache pressures; solid
+    // programs list the union of current and reference neighbourhoods (needs
+    // the reference positions).  A list stays valid until positions,
+    // reference positions or the slot order change.
+    const bool solid_program = !h->kset.fluids && (h->kset.solids || h->kset.corot || h->kset.svk);
+    const bool listed = h->pairs.list != nullptr && (h->kset.fluids || (solid_program && h->field("originalpos")));
+    if (internal == "compute_density" || internal == "step_forward" || internal == "apply_plane_constraints" ||
+        internal == "init_original_position" || internal == "find_particle_bins")
+        h->pairs.valid = 0;
     const sphk::PairCache cache = h->pairs;
-    if (internal != "compute_forces_fluids") h->pairs.valid = 0;
-    if (internal == "compute_density" && listed)
+    if (internal == "compute_density" && listed && h->kset.fluids)
         rc = timed(h, "compute_density",
                    [&] { return sphk::launch_compute_density_listed(p, f, cache, s); });
+    else if (internal == "compute_density" && listed)
+        rc = timed(h, "compute_density",
+                   [&] { return sphk::launch_compute_density_solid_listed(p, f, cache, s); });
     else if (internal == "compute_density")
         rc = timed(h, "compute_density", [&] { return sphk::launch_compute_density(p, f, s); });
-    else if (internal == "compute_forces_fluids" && listed && cache.valid)
+    else if (internal == "compute_forces_fluids" && cache.valid == sphk::kFluidList)
         rc = timed(h, "compute_forces_fluids",
                    [&] { return sphk::launch_forces_fluids_listed(p, f, cache, s); });
     else if (internal == "compute_forces_fluids")
@@ -457,17 +467,20 @@ int run_kernel(psdata_hip *h, const std::string &internal) {
     else if (internal == "compute_original_density")
         rc = timed(h, "compute_original_density", [&] { return sphk::launch_compute_original_density(p, f, s); });
     else if (internal == "compute_rotations_and_strains")
-        rc = timed(h, "compute_rotations_and_strains", [&] { return sphk::launch_rotations_and_strains(p, f, s); });
+        rc = timed(h, "compute_rotations_and_strains",
+                   [&] { return sphk::launch_rotations_and_strains(p, f, cache, s); });
     else if (internal == "compute_stresses_corotational")
         rc = timed(h, "compute_stresses_corotational", [&] { return sphk::launch_stresses_corotational(p, f, s); });
     else if (internal == "compute_deformations")
-        rc = timed(h, "compute_deformations", [&] { return sphk::launch_deformations_svk(p, f, s); });
+        rc = timed(h, "compute_deformations", [&] { return sphk::launch_deformations_svk(p, f, cache, s); });
     else if (internal == "compute_stresses_svk")
         rc = timed(h, "compute_stresses_svk", [&] { return sphk::launch_stresses_svk(p, f, s); });
     else if (internal == "compute_forces_solids_corotational")
-        rc = timed(h, "compute_forces_solids_corotational", [&] { return sphk::launch_forces_solids(p, f, true, s); });
+        rc = timed(h, "compute_forces_solids_corotational",
+                   [&] { return sphk::launch_forces_solids(p, f, cache, true, s); });
     else if (internal == "compute_forces_solids_svk")
-        rc = timed(h, "compute_forces_solids_svk", [&] { return sphk::launch_forces_solids(p, f, false, s); });
+        rc = timed(h, "compute_forces_solids_svk",
+                   [&] { return sphk::launch_forces_solids(p, f, cache, false, s); });
     else if (internal == "find_particle_bins")
         rc = timed(h, "find_particle_bins", [&] {
             return sphk::launch_cell_keys(f.position, p.n, p.grid, f.gridcell, s);
@@ -476,7 +489,7 @@ int run_kernel(psdata_hip *h, const std::string &internal) {
         return fail("unknown kernel '%s'", internal.c_str());
     if (rc) return -1;
     mark_written(h, k);
-    if (internal == "compute_density" && listed) h->pairs.valid = 1;
+    if (internal == "compute_density" && listed) h->pairs.valid = h->kset.fluids ? sphk::kFluidList : sphk::kSolidList;
     return 0;
 }
 
@@ -889,7 +902,8 @@ extern "C" int create_psdata_hip(psdata *data, const char *file_list, psdata_hip
     if (hipMalloc(reinterpret_cast<void **>(&h->bins.tile_sums), tiles * 4) != hipSuccess ||
         hipMalloc(&h->staging, max_particle_bytes) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&h->sums), (7 * sphk::kReduceBlocks + 8) * sizeof(double)) != hipSuccess ||
-        (h->kset.fluids && h->has_grid &&
+        ((h->kset.fluids || ((h->kset.solids || h->kset.corot || h->kset.svk) && h->pnum <= sphk::kListSlot)) &&
+         h->has_grid &&
          (hipMalloc(reinterpret_cast<void **>(&h->pairs.list), pn4 * sphk::kListCap) != hipSuccess ||
           hipMalloc(reinterpret_cast<void **>(&h->pairs.count), pn4) != hipSuccess ||
           hipMalloc(reinterpret_cast<void **>(&h->pairs.pressure), pn4) != hipSuccess))) {
