@@ -223,7 +223,7 @@ def run_single(args):
 def _brief(rep):
     keep = ("requests", "particle_steps", "wall_s", "particle_steps_per_s", "latency_p50_s", "latency_p99_s",
             "latency_mean_s", "latency_max_s", "peak_resident_slots", "peak_resident_scenes", "batch_steps",
-            "graph_captures", "graph_launches", "pool_bytes")
+            "graph_captures", "graph_launches", "pool_bytes", "host_phases_s")
     return {k: (round(v, 5) if isinstance(v, float) else v) for k, v in rep.items() if k in keep}
 
 

@@ -110,6 +110,7 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
     done = {}
     busy_steps = 0
     peak_slots = peak_scenes = 0
+    phase = {"generate": 0.0, "admit": 0.0, "enqueue": 0.0, "wait": 0.0, "read": 0.0, "idle": 0.0}
     t0 = time.perf_counter()
     clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
     try:
@@ -124,13 +125,17 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
             for k, item in enumerate(waiting[:window]):
                 req, scene = item
                 if scene is None:
+                    tg = time.perf_counter()
                     scene = item[1] = req.scene()
+                    phase["generate"] += time.perf_counter() - tg
                 if not batch.can_admit(scene):
                     misses += 1
                     if misses >= 4:        # the pool is close to full
                         break
                     continue
+                ta = time.perf_counter()
                 handle = batch.add(scene, req.steps)
+                phase["admit"] += time.perf_counter() - ta
                 if handle is None:
                     continue
                 running[handle] = (req, clock() if realtime else 0.0)
@@ -138,22 +143,31 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
             waiting = [item for k, item in enumerate(waiting) if k not in admitted]
             if not running:
                 if pending:
+                    ti = time.perf_counter()
                     time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.05)))
+                    phase["idle"] += time.perf_counter() - ti
                 continue
             info = batch.info()
             peak_slots = max(peak_slots, info["used_slots"])
             peak_scenes = max(peak_scenes, info["scenes"])
+            te = time.perf_counter()
             batch.step(chunk)
             busy_steps += chunk
+            phase["enqueue"] += time.perf_counter() - te
             # generate scenes of requests that arrived meanwhile while the
             # device works
             now = clock()
             while pending and pending[0].arrival <= now:
                 waiting.append([pending.popleft(), None])
+            tg = time.perf_counter()
             for item in waiting[:8]:
                 if item[1] is None:
                     item[1] = item[0].scene()
+            phase["generate"] += time.perf_counter() - tg
+            tw = time.perf_counter()
             batch.synchronize()
+            phase["wait"] += time.perf_counter() - tw
+            tr = time.perf_counter()
             for handle in [h for h in running if batch.steps_left(h) == 0]:
                 req, _ = running.pop(handle)
                 result = batch.read(handle, *fields)
@@ -162,6 +176,7 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
                 done[req.index] = {"latency": t_done - (req.arrival if realtime else 0.0), "finished": t_done}
                 if on_result:
                     on_result(req, result)
+            phase["read"] += time.perf_counter() - tr
             if log and busy_steps % (chunk * 100) == 0:
                 log(f"batched: {len(done)}/{len(requests)} done, {len(running)} resident, "
                     f"{len(waiting)} waiting, t={time.perf_counter() - t0:.1f}s")
@@ -177,7 +192,7 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
                      "peak_resident_slots": peak_slots, "peak_resident_scenes": peak_scenes,
                      "batch_steps": busy_steps, "graph_captures": info["graph_captures"],
                      "graph_launches": info["graph_launches"], "graph_steps": graph_steps,
-                     "kernel_stats": stats})
+                     "kernel_stats": stats, "host_phases_s": {k: round(v, 4) for k, v in phase.items()}})
 
 
 def serve_plain(requests, compensated_sums=1, polar_fp64=True, realtime=True, on_result=None, fields=("position",),
