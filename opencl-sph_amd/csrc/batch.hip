@@ -15,13 +15,15 @@
 // to the scene's slots.  A scene therefore computes the same bits in a batch
 // as on its own, at the same precision settings.
 //
-// Stepping.  One step = bin (key/count, padding, scan, scatter, rank, one
-// gather of the persistent per-slot state), then density, the force passes of
-// each system present, the integrator and a per-scene step counter.  Scenes
-// stop on their own count; the host mirrors the counters, so it knows which
-// scenes are finished without reading the device.  The K-step sequence can
-// be captured into a hipGraph and replayed while the pool's layout is
-// unchanged.
+// Stepping.  One step = bin every resident scene (key/count with the
+// padding cells, scan, scatter, rank with the gather of the persistent
+// per-slot state), then per system, on parallel streams, the density and
+// force passes over that system's chunk map, then the integrator and a
+// per-scene step counter.  Scenes stop on their own count (a finished scene
+// is still binned, which keeps its slots' order, until it is removed); the
+// host mirrors the counters, so it knows which scenes are finished without
+// reading the device.  The K-step sequence can be captured into a hipGraph
+// and replayed while the pool's layout is unchanged.
 //
 // Counterpart in the reference: none (the reference runs one psdata per
 // OpenCL program, src/opencl/particle_system_host.c:361-528); the per-scene
@@ -60,12 +62,36 @@ struct SlotState {
 
 __device__ __forceinline__ unsigned linear_slot() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
+// Slot i holds a particle of a resident scene (finished or not).
+__device__ __forceinline__ bool live(const SceneTable &ps, unsigned i) {
+    if (i >= ps.t.capacity) return false;
+    const unsigned s = ps.t.scene_of[i];
+    if (s == kNoScene) return false;
+    const StepParams &p = ps.t.scenes[s];
+    return i < p.p0 + p.n;
+}
+// ... of a scene with steps left: those are sorted; the slots of finished
+// scenes keep their order (their outputs stay where they were computed).
+__device__ __forceinline__ bool binned(const SceneTable &ps, unsigned i) {
+    return live(ps, i) && ps.t.steps_left[ps.t.scene_of[i]] > 0u;
+}
+
+// Cell key and arrival rank of every binned slot; threads k < nsegs also
+// set the padding cell of segment k to the number of its slots that are not
+// binned.
 __global__ void __launch_bounds__(kBlock) k_batch_key_count(SceneTable ps, const float *__restrict__ pos,
+                                                            const Segment *__restrict__ segs, unsigned nsegs,
                                                             unsigned *__restrict__ key,
                                                             unsigned *__restrict__ local,
                                                             unsigned *__restrict__ count) {
     const unsigned s = linear_slot();
-    const bool active = ps.active(s);
+    if (s < nsegs) {
+        const Segment g = segs[s];
+        const unsigned used =
+            g.scene != kNoScene && ps.t.steps_left[g.scene] > 0u ? ps.t.scenes[g.scene].n : 0u;
+        count[g.c0 + g.cellcap - 1u] = g.cap - used;
+    }
+    const bool active = binned(ps, s);
     unsigned k = 0xffffffffu;
     if (active) {
         const StepParams &p = ps.at(s);
@@ -77,17 +103,6 @@ __global__ void __launch_bounds__(kBlock) k_batch_key_count(SceneTable ps, const
     local[s] = rank;
 }
 
-__global__ void k_batch_padding(const Segment *__restrict__ segs, unsigned nsegs,
-                                const StepParams *__restrict__ scenes, const unsigned *__restrict__ steps_left,
-                                unsigned *__restrict__ count) {
-    const unsigned k = linear_slot();
-    if (k >= nsegs) return;
-    const Segment g = segs[k];
-    unsigned binned = 0;
-    if (g.scene != kNoScene && steps_left[g.scene] > 0u) binned = scenes[g.scene].n;
-    count[g.c0 + g.cellcap - 1u] = g.cap - binned;
-}
-
 __global__ void __launch_bounds__(kBlock) k_batch_scatter(SceneTable ps, const unsigned *__restrict__ key,
                                                           const unsigned *__restrict__ local,
                                                           const unsigned *__restrict__ celloffset,
@@ -96,32 +111,12 @@ __global__ void __launch_bounds__(kBlock) k_batch_scatter(SceneTable ps, const u
                                                           unsigned *__restrict__ tmp_src,
                                                           unsigned *__restrict__ tmp_key) {
     const unsigned s = linear_slot();
-    if (!ps.active(s)) return;
+    if (!binned(ps, s)) return;
     const unsigned k = key[s];
     const unsigned t = celloffset[k] + local[s];
     tmp_id[t] = id[s];
     tmp_src[t] = s;
     tmp_key[t] = k;
-}
-
-// The binned slots of a scene after the sort are exactly its active slots
-// [p0, p0 + n), so `active` selects the filled unsorted positions too.
-__global__ void __launch_bounds__(kBlock) k_batch_rank(SceneTable ps, const unsigned *__restrict__ tmp_id,
-                                                       const unsigned *__restrict__ tmp_src,
-                                                       const unsigned *__restrict__ tmp_key,
-                                                       const unsigned *__restrict__ celloffset,
-                                                       const unsigned *__restrict__ gridcount,
-                                                       unsigned *__restrict__ perm, unsigned *__restrict__ gridcell) {
-    const unsigned t = linear_slot();
-    if (!ps.active(t)) return;
-    const unsigned c = tmp_key[t];
-    const unsigned start = celloffset[c], cnt = gridcount[c];
-    const unsigned me = tmp_id[t];
-    unsigned rank = 0;
-    for (unsigned u = start; u < start + cnt; ++u) rank += tmp_id[u] < me ? 1u : 0u;
-    const unsigned f = start + rank;
-    perm[f] = tmp_src[t];
-    gridcell[f] = c;
 }
 
 template <unsigned W, typename T>
@@ -131,19 +126,35 @@ __device__ __forceinline__ void copy_words(const T *__restrict__ src, T *__restr
     for (unsigned w = 0; w < W; ++w) dst[static_cast<size_t>(to) * W + w] = src[static_cast<size_t>(from) * W + w];
 }
 
-// New slot f takes the state of old slot perm[f] (binned slots) or keeps its
-// own (slots of scenes with no steps left).  Free slots are not touched.
-__global__ void __launch_bounds__(kBlock) k_batch_gather(SceneTable ps, const unsigned *__restrict__ perm,
-                                                         SlotState a, SlotState b) {
-    const unsigned f = linear_slot();
-    if (f >= ps.t.capacity) return;
-    const unsigned s = ps.t.scene_of[f];
-    if (s == kNoScene) return;
-    const unsigned from = ps.active(f) ? perm[f] : f;
-    const unsigned system = ps.t.scenes[s].system;
+// Stable placement (ascending particle id inside each cell) and the move of
+// the persistent state of each particle into its new slot.  The unsorted
+// positions of a scene's particles after the scatter are exactly its slots
+// [p0, p0 + n), so `binned` selects them; slots of finished scenes copy
+// their state in place.
+__global__ void __launch_bounds__(kBlock) k_batch_rank_gather(SceneTable ps, const unsigned *__restrict__ tmp_id,
+                                                              const unsigned *__restrict__ tmp_src,
+                                                              const unsigned *__restrict__ tmp_key,
+                                                              const unsigned *__restrict__ celloffset,
+                                                              const unsigned *__restrict__ gridcount,
+                                                              unsigned *__restrict__ gridcell, SlotState a,
+                                                              SlotState b) {
+    const unsigned t = linear_slot();
+    if (!live(ps, t)) return;
+    unsigned f = t, from = t;
+    if (ps.t.steps_left[ps.t.scene_of[t]] > 0u) {
+        const unsigned c = tmp_key[t];
+        const unsigned start = celloffset[c], cnt = gridcount[c];
+        const unsigned me = tmp_id[t];
+        unsigned rank = 0;
+        for (unsigned u = start; u < start + cnt; ++u) rank += tmp_id[u] < me ? 1u : 0u;
+        f = start + rank;
+        from = tmp_src[t];
+        gridcell[f] = c;
+    }
+    copy_words<1>(a.id, b.id, from, f);
+    const unsigned system = ps.at(t).system;
     copy_words<3>(a.pos, b.pos, from, f);
     copy_words<3>(a.vel, b.vel, from, f);
-    copy_words<1>(a.id, b.id, from, f);
     if (system != kFluid) {
         copy_words<3>(a.opos, b.opos, from, f);
         copy_words<1>(a.d0, b.d0, from, f);
@@ -193,6 +204,8 @@ struct BatchTiming {
 
 struct sph_batch {
     hipStream_t stream = nullptr;
+    hipStream_t side[2] = {nullptr, nullptr};   // corotational / svk branches of a step
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     int device = 0;
     unsigned slot_cap = 0, cell_cap = 0, scene_cap = 0;
     int compensated = 0, polar_fp64 = 1;
@@ -202,7 +215,7 @@ struct sph_batch {
     SlotState st[2]{};
     int cur = 0;
     float *force = nullptr, *density = nullptr, *stress = nullptr, *deformation = nullptr;
-    unsigned *gridcell = nullptr, *scene_of = nullptr, *perm = nullptr;
+    unsigned *gridcell = nullptr, *scene_of = nullptr;
     sphk::BinScratch bins{};
     sphk::PairCache pairs{};
     void *staging = nullptr;
@@ -212,9 +225,9 @@ struct sph_batch {
     unsigned *d_steps_left = nullptr, *body_dynamic = nullptr;
     Segment *d_segs = nullptr;
 
-    // per-system chunk maps (fluid, corotational, svk, both solids): the
-    // 64-slot chunks of the running scenes of each, rebuilt on layout changes
-    static constexpr int kMaps = 4;
+    // per-system chunk maps (fluid, corotational, svk): the 64-slot chunks of
+    // the running scenes of each, rebuilt on layout changes
+    static constexpr int kMaps = 3;
     unsigned *d_chunk_slot[kMaps] = {}, *d_chunk_end[kMaps] = {};
     unsigned nchunks[kMaps] = {};
     size_t max_chunks = 0;
@@ -242,6 +255,7 @@ struct sph_batch {
     hipGraphExec_t graph = nullptr;
     unsigned long long graph_version = 0;
     int graph_cur = 0;                     // state buffer parity the graph starts from
+    unsigned long long eager_version = 0;  // layout last stepped without a graph
     unsigned long long graph_launches = 0, graph_captures = 0;
 };
 
@@ -261,11 +275,11 @@ int dev_alloc(sph_batch *b, T **p, size_t count) {
 size_t pool_bytes(size_t slots, size_t cells, size_t scenes) {
     const size_t per_slot = 2 * (3 + 3 + 3 + 1 + 9 + 6 + 1 + 1) * 4   // double-buffered state
                             + (3 + 1 + 6 + 9) * 4                      // force, density, stress, deformation
-                            + 3 * 4                                    // gridcell, scene_of, perm
+                            + 2 * 4                                    // gridcell, scene_of
                             + 5 * 4                                    // bin scratch
                             + (sphk::kListCap + 2) * 4                 // pair cache
                             + 9 * 4                                    // readback staging
-                            + 1;                                       // chunk maps (4 x 8 B per 64 slots)
+                            + 1;                                       // chunk maps (3 x 8 B per 64 slots)
     const size_t per_cell = 2 * 4;
     const size_t tiles = (cells + sphk::kScanTile - 1) / sphk::kScanTile + 1;
     const size_t per_scene = sizeof(StepParams) + 4 + sphk::kBodiesPerScene * 4 + sizeof(Segment) * 2 + 4 * 8;
@@ -281,6 +295,10 @@ void release(sph_batch *b) {
         hipEventDestroy(t.stop);
     }
     for (hipEvent_t e : b->event_pool) hipEventDestroy(e);
+    for (hipEvent_t e : {b->ev_fork, b->ev_join[0], b->ev_join[1]})
+        if (e) hipEventDestroy(e);
+    for (hipStream_t q : b->side)
+        if (q) hipStreamDestroy(q);
     for (void *p : b->allocations) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -290,7 +308,7 @@ sphk::BatchTable table(const sph_batch *b) {
     return sphk::BatchTable{b->d_scenes, b->scene_of, b->d_steps_left, b->used_slots, nullptr, nullptr, 0};
 }
 
-enum MapIndex { kMapFluid = 0, kMapCorot = 1, kMapSvk = 2, kMapSolids = 3 };
+enum MapIndex { kMapFluid = 0, kMapCorot = 1, kMapSvk = 2 };
 
 // The scene table restricted to the chunks of one system's running scenes.
 sphk::BatchTable table(const sph_batch *b, int map) {
@@ -312,10 +330,6 @@ int build_chunk_maps(sph_batch *b) {
         for (unsigned c = 0; c < p.n; c += sphk::kChunk) {
             slot[m].push_back(p.p0 + c);
             end[m].push_back(p.p0 + p.n);
-            if (m != kMapFluid) {
-                slot[kMapSolids].push_back(p.p0 + c);
-                end[kMapSolids].push_back(p.p0 + p.n);
-            }
         }
     }
     for (int m = 0; m < sph_batch::kMaps; ++m) {
@@ -440,10 +454,8 @@ int enqueue_bins(sph_batch *b) {
     if (timed(b, "batch_key_count", [&] {
             hipError_t e = sphk::launch_fill_u32(b->gridcount, b->used_cells, 0u, s);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(sphk::k_batch_padding, sphk::blocks(nsegs), block, 0, s, b->d_segs, nsegs,
-                               b->d_scenes, b->d_steps_left, b->gridcount);
-            hipLaunchKernelGGL(sphk::k_batch_key_count, grid, block, 0, s, all, a.pos, b->bins.key, b->bins.local,
-                               b->gridcount);
+            hipLaunchKernelGGL(sphk::k_batch_key_count, sphk::blocks(std::max<size_t>(b->used_slots, nsegs)), block,
+                               0, s, all, a.pos, b->d_segs, nsegs, b->bins.key, b->bins.local, b->gridcount);
             return hipGetLastError();
         }) ||
         timed(b, "batch_scan", [&] {
@@ -454,13 +466,9 @@ int enqueue_bins(sph_batch *b) {
                                b->celloffset, a.id, b->bins.tmp_id, b->bins.tmp_src, b->bins.tmp_key);
             return hipGetLastError();
         }) ||
-        timed(b, "batch_rank", [&] {
-            hipLaunchKernelGGL(sphk::k_batch_rank, grid, block, 0, s, all, b->bins.tmp_id, b->bins.tmp_src,
-                               b->bins.tmp_key, b->celloffset, b->gridcount, b->perm, b->gridcell);
-            return hipGetLastError();
-        }) ||
-        timed(b, "batch_gather", [&] {
-            hipLaunchKernelGGL(sphk::k_batch_gather, grid, block, 0, s, all, b->perm, a, n);
+        timed(b, "batch_rank_gather", [&] {
+            hipLaunchKernelGGL(sphk::k_batch_rank_gather, grid, block, 0, s, all, b->bins.tmp_id, b->bins.tmp_src,
+                               b->bins.tmp_key, b->celloffset, b->gridcount, b->gridcell, a, n);
             return hipGetLastError();
         }))
         return -1;
@@ -468,17 +476,21 @@ int enqueue_bins(sph_batch *b) {
     return 0;
 }
 
-// One step of every scene with steps left.  `systems` selects the force
-// passes to launch (bit SystemKind); `setup` adds the reference-configuration
-// passes for scenes flagged kSceneSetup.
+// One step of every scene with steps left.  `systems` selects the passes to
+// launch (bit SystemKind); `setup` adds the reference-configuration passes
+// for scenes flagged kSceneSetup.  The three systems' passes touch disjoint
+// slots, so unless per-pass timing is on they run on three streams, forked
+// after binning and joined before the integrator (parallel branches of a
+// captured graph).
 int enqueue_step(sph_batch *b, unsigned systems, bool setup) {
     hipStream_t s = b->stream;
+    const bool branch = !b->profiling;
+    hipStream_t sc = branch ? b->side[0] : s, sv = branch ? b->side[1] : s;
     const sphk::BatchTable t = table(b);
     const sphk::BatchTable tf = table(b, kMapFluid), tc = table(b, kMapCorot), tv = table(b, kMapSvk);
-    const sphk::BatchTable ts = table(b, kMapSolids);
     const int kahan = b->compensated, fp64 = b->polar_fp64;
     const unsigned solid_mask = (1u << sphk::kCorotational) | (1u << sphk::kSvk);
-    const sphk::SceneFilter fluid{1u << sphk::kFluid, 0u}, solids{solid_mask, 0u};
+    const sphk::SceneFilter fluid{1u << sphk::kFluid, 0u};
     const sphk::SceneFilter corot{1u << sphk::kCorotational, 0u}, svk{1u << sphk::kSvk, 0u};
     const sphk::SceneFilter setup_sel{solid_mask, sphk::kSceneSetup};
     if (setup && timed(b, "init_original_position", [&] {
@@ -490,33 +502,49 @@ int enqueue_step(sph_batch *b, unsigned systems, bool setup) {
     if (setup && timed(b, "compute_original_density",
                        [&] { return sphk::launch_batch_original_density(t, setup_sel, f, kahan, s); }))
         return -1;
+    const bool has_c = (systems >> sphk::kCorotational) & 1u, has_v = (systems >> sphk::kSvk) & 1u;
+    if (branch) {
+        SPH_HIP_TRY(hipEventRecord(b->ev_fork, s));
+        if (has_c) SPH_HIP_TRY(hipStreamWaitEvent(sc, b->ev_fork, 0));
+        if (has_v) SPH_HIP_TRY(hipStreamWaitEvent(sv, b->ev_fork, 0));
+    }
+    sphk::PairCache solid_pairs = b->pairs;
+    solid_pairs.valid = sphk::kSolidList;
+    if (has_c &&
+        (timed(b, "compute_density_solid",
+               [&] { return sphk::launch_batch_density_solid_listed(tc, corot, f, solid_pairs, kahan, sc); }) ||
+         timed(b, "compute_rotations_and_strains",
+               [&] { return sphk::launch_batch_rotations_and_strains(tc, corot, f, solid_pairs, kahan, fp64, sc); }) ||
+         timed(b, "compute_stresses_corotational",
+               [&] { return sphk::launch_batch_stresses_corotational(tc, corot, f, sc); }) ||
+         timed(b, "compute_forces_solids_corotational",
+               [&] { return sphk::launch_batch_forces_solids(tc, corot, f, solid_pairs, true, kahan, sc); })))
+        return -1;
+    if (has_v &&
+        (timed(b, "compute_density_solid",
+               [&] { return sphk::launch_batch_density_solid_listed(tv, svk, f, solid_pairs, kahan, sv); }) ||
+         timed(b, "compute_deformations",
+               [&] { return sphk::launch_batch_deformations_svk(tv, svk, f, solid_pairs, kahan, fp64, sv); }) ||
+         timed(b, "compute_stresses_svk", [&] { return sphk::launch_batch_stresses_svk(tv, svk, f, sv); }) ||
+         timed(b, "compute_forces_solids_svk",
+               [&] { return sphk::launch_batch_forces_solids(tv, svk, f, solid_pairs, false, kahan, sv); })))
+        return -1;
     if ((systems & (1u << sphk::kFluid)) &&
         (timed(b, "compute_density_fluid",
                [&] { return sphk::launch_batch_density_listed(tf, fluid, f, b->pairs, kahan, s); }) ||
          timed(b, "compute_forces_fluids",
                [&] { return sphk::launch_batch_forces_fluids_listed(tf, fluid, f, b->pairs, kahan, s); })))
         return -1;
-    sphk::PairCache solid_pairs = b->pairs;
-    solid_pairs.valid = sphk::kSolidList;
-    if ((systems & solid_mask) && timed(b, "compute_density_solid", [&] {
-            return sphk::launch_batch_density_solid_listed(ts, solids, f, solid_pairs, kahan, s);
-        }))
-        return -1;
-    if ((systems & (1u << sphk::kCorotational)) &&
-        (timed(b, "compute_rotations_and_strains",
-               [&] { return sphk::launch_batch_rotations_and_strains(tc, corot, f, solid_pairs, kahan, fp64, s); }) ||
-         timed(b, "compute_stresses_corotational",
-               [&] { return sphk::launch_batch_stresses_corotational(tc, corot, f, s); }) ||
-         timed(b, "compute_forces_solids_corotational",
-               [&] { return sphk::launch_batch_forces_solids(tc, corot, f, solid_pairs, true, kahan, s); })))
-        return -1;
-    if ((systems & (1u << sphk::kSvk)) &&
-        (timed(b, "compute_deformations",
-               [&] { return sphk::launch_batch_deformations_svk(tv, svk, f, solid_pairs, kahan, fp64, s); }) ||
-         timed(b, "compute_stresses_svk", [&] { return sphk::launch_batch_stresses_svk(tv, svk, f, s); }) ||
-         timed(b, "compute_forces_solids_svk",
-               [&] { return sphk::launch_batch_forces_solids(tv, svk, f, solid_pairs, false, kahan, s); })))
-        return -1;
+    if (branch) {
+        if (has_c) {
+            SPH_HIP_TRY(hipEventRecord(b->ev_join[0], sc));
+            SPH_HIP_TRY(hipStreamWaitEvent(s, b->ev_join[0], 0));
+        }
+        if (has_v) {
+            SPH_HIP_TRY(hipEventRecord(b->ev_join[1], sv));
+            SPH_HIP_TRY(hipStreamWaitEvent(s, b->ev_join[1], 0));
+        }
+    }
     if (timed(b, "step_forward", [&] { return sphk::launch_batch_step_forward(t, sphk::kAllScenes, f, s); }) ||
         timed(b, "batch_tick", [&] {
             hipLaunchKernelGGL(sphk::k_batch_tick, sphk::blocks(b->scene_cap), dim3(sphk::kBlock), 0, s,
@@ -689,7 +717,7 @@ extern "C" int sph_batch_create(unsigned slots, unsigned cells, unsigned scenes,
     float *pair_pressure = nullptr;
     rc = rc || dev_alloc(b, &b->force, 3 * S) || dev_alloc(b, &b->density, S) || dev_alloc(b, &b->stress, 6 * S) ||
          dev_alloc(b, &b->deformation, 9 * S) || dev_alloc(b, &b->gridcell, S) || dev_alloc(b, &b->scene_of, S) ||
-         dev_alloc(b, &b->perm, S) || dev_alloc(b, &b->bins.key, S) || dev_alloc(b, &b->bins.local, S) ||
+         dev_alloc(b, &b->bins.key, S) || dev_alloc(b, &b->bins.local, S) ||
          dev_alloc(b, &b->bins.tmp_id, S) || dev_alloc(b, &b->bins.tmp_src, S) || dev_alloc(b, &b->bins.tmp_key, S) ||
          dev_alloc(b, &pair_list, sphk::kListCap * S) || dev_alloc(b, &pair_count, S) ||
          dev_alloc(b, &pair_pressure, S) || dev_alloc(b, reinterpret_cast<unsigned **>(&b->staging), 9 * S) ||
@@ -705,9 +733,14 @@ extern "C" int sph_batch_create(unsigned slots, unsigned cells, unsigned scenes,
         return -1;
     }
     b->pairs = sphk::PairCache{pair_list, pair_count, pair_pressure, slots, 1};
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side[0], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side[1], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join[1], hipEventDisableTiming) != hipSuccess) {
         release(b);
-        return fail("hipStreamCreate failed");
+        return fail("stream / event creation failed");
     }
     b->scenes.assign(scenes, StepParams{});
     b->steps_left.assign(scenes, 0u);
@@ -863,7 +896,10 @@ extern "C" int sph_batch_step(sph_batch *b, unsigned num_steps) {
             // the upload reads host memory that the next step may change
             SPH_HIP_TRY(hipStreamSynchronize(b->stream));
             done += 1;
-        } else if (b->graph_steps > 0 && !b->profiling && num_steps - done >= b->graph_steps) {
+        } else if (b->graph_steps > 0 && !b->profiling && num_steps - done >= b->graph_steps &&
+                   (b->graph_version == b->layout_version || b->eager_version == b->layout_version)) {
+            // A layout is captured once it has run one round of steps without
+            // a graph: layouts that change every round are not worth a capture.
             if (!b->graph || b->graph_version != b->layout_version || b->graph_cur != b->cur) {
                 if (b->graph) {
                     SPH_HIP_TRY(hipGraphExecDestroy(b->graph));
@@ -898,6 +934,7 @@ extern "C" int sph_batch_step(sph_batch *b, unsigned num_steps) {
         } else {
             if (enqueue_step(b, systems_present(b), false)) return -1;
             done += 1;
+            if (done >= b->graph_steps) b->eager_version = b->layout_version;
         }
     }
     // mirror of the device step counters for the steps just enqueued

@@ -85,7 +85,7 @@ def test_batch_matches_scenes_alone(graph_steps, compensated):
         info = batch.info()
         say(f"  graph captures {info['graph_captures']}, launches {info['graph_launches']}")
         if graph_steps:
-            assert info["graph_launches"] >= 2
+            assert info["graph_launches"] >= 1
     finally:
         batch.free()
     for k in sc:
