@@ -57,25 +57,36 @@ def grid_pair_counts(system, scene):
     return candidates, in_range
 
 
-# Timed kernel name -> device symbol prefix in the rocprofv3 counter summaries.
-KERNEL_SYMBOLS = {"compute_density": "k_density_listed", "compute_forces_fluids": "k_forces_fluids_listed",
-                  "bin_rank_place": "k_rank_place", "step_forward": "k_step_forward",
+# Timed pass name -> device symbol prefix in the rocprofv3 counter summaries
+# (a pass may launch several kernels with that prefix: their traffic adds up).
+KERNEL_SYMBOLS = {"compute_density": "k_density_listed<sphk::SingleScene",
+                  "compute_forces_fluids": "k_forces_fluids_listed<sphk::SingleScene",
+                  "bin_rank_place": "k_rank_place", "step_forward": "k_step_forward<sphk::SingleScene",
                   "bin_scatter": "k_scatter_unsorted", "bin_key_count": "k_key_count"}
+BATCH_KERNEL_SYMBOLS = {
+    "compute_density_fluid": "k_density_listed<sphk::SceneTable",
+    "compute_forces_fluids": "k_forces_fluids_listed<sphk::SceneTable",
+    "compute_density_solid": "k_density_solid_listed<sphk::SceneTable",
+    "compute_rotations_and_strains": "k_rotations_strains<sphk::SceneTable",
+    "compute_deformations": "k_deformations_svk<sphk::SceneTable",
+    "compute_forces_solids_corotational": "k_forces_solids<sphk::SceneTable, true",
+    "compute_forces_solids_svk": "k_forces_solids<sphk::SceneTable, false",
+    "step_forward": "k_step_forward<sphk::SceneTable", "batch_rank_gather": "k_batch_rank_gather",
+}
 
 
 def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` from the calibrated FETCH_SIZE /
+    """HBM bytes per launch of pass `kernel` from the calibrated FETCH_SIZE /
     WRITE_SIZE summary committed under profiles/, or None."""
     path = os.path.join(REPO, "profiles", f"{workload}_hbm_traffic.json")
-    prefix = KERNEL_SYMBOLS.get(kernel)
+    table = BATCH_KERNEL_SYMBOLS if workload == "served_stream" else KERNEL_SYMBOLS
+    prefix = table.get(kernel)
     if prefix is None or not os.path.exists(path):
         return None
     with open(path) as f:
         summary = json.load(f)
-    for name, row in summary.get("kernels", {}).items():
-        if name.startswith(prefix):
-            return int(row["read_bytes"] + row["write_bytes"])
-    return None
+    rows = [row for name, row in summary.get("kernels", {}).items() if name.startswith(prefix)]
+    return int(sum(r["read_bytes"] + r["write_bytes"] for r in rows)) if rows else None
 
 
 def copy_probe(mib):
@@ -145,8 +156,6 @@ def run_single(args):
     import sph
     from sph import roofline, scenes
 
-    if args.copy_probe_mib > 0:
-        copy_probe(args.copy_probe_mib)
     scene = scenes.baseline_config(WORKLOADS[args.workload])
     log(f"workload {scene.name}: {scene.n} particles, system {scene.system}")
     system = scenes.build(scene)
@@ -237,6 +246,14 @@ def run_served(args):
     budget = args.hbm_budget_gb * 1e9
     log(f"served stream: {len(reqs)} requests, {steps} steps each, "
         f"{sum(r.n * r.steps for r in reqs):.4g} particle-steps, last arrival {reqs[-1].arrival:.1f} s")
+
+    if args.counters_only:
+        # Only the profiled pass, for hardware-counter runs under rocprofv3.
+        prof = serve.serve_batched(reqs[:args.profile_requests], hbm_budget=budget, realtime=False,
+                                   compensated_sums=sums, profile=True)
+        return {"counters_only": True, "requests": args.profile_requests,
+                "particle_steps_by_system": prof["particle_steps_by_system"],
+                "kernel_stats": prof["kernel_stats"]}
 
     # warm-up: both paths on a few requests of another seed
     warm = serve.make_requests(count=max(args.warmup, 1), rate=args.rate, seed=args.seed + 1, steps=4)
@@ -330,6 +347,8 @@ def main():
     ap.add_argument("--profile-requests", type=int, default=64, help="served stream: prefix of the profiled pass")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0, help="served stream: admission budget")
     ap.add_argument("--graph-steps", type=int, default=8, help="served stream: batch steps per hipGraph (even)")
+    ap.add_argument("--counters-only", action="store_true",
+                    help="served stream: run only the profiled pass (for rocprofv3 counter collection)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the engine runs one scene set per device and this build has no "
@@ -344,6 +363,8 @@ def main():
 
     bindings.set_log_level(3)          # library notes would share stdout with the JSON line
     sph.init_hip()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
     line = run_served(args) if args.workload == "served_stream" else run_single(args)
     print(json.dumps(line), flush=True)
 

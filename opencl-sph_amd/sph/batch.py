@@ -61,20 +61,44 @@ class Batch:
         """Whether `scene` would be admitted now."""
         return self._batch.can_admit(*self.footprint(scene))
 
-    def add(self, scene, steps=None):
-        """Admit `scene` for `steps` steps (default scene.steps).  Returns its
-        handle, or None when the pool has no room for it now."""
+    @staticmethod
+    def prepare(scene):
+        """Host side of an admission: the scene's psdata with its initial
+        fields.  Touches no device state, so it may run on another thread
+        while the device steps; pass the result to add_prepared (or
+        discard_prepared)."""
         psdata = init_data_cpu(scene.spec)
         try:
             fill_fields(psdata, scene.fields)
-            handle = self._batch.add(psdata, scene.kernel_files, scene.steps if steps is None else steps)
         except Exception:
             free_data_cpu(psdata)
             raise
+        return psdata, scene
+
+    @staticmethod
+    def discard_prepared(prepared):
+        free_data_cpu(prepared[0])
+
+    def add_prepared(self, prepared, steps=None):
+        """Admit a prepared scene; returns its handle, or None when the pool
+        has no room for it now (the prepared scene stays valid)."""
+        psdata, scene = prepared
+        handle = self._batch.add(psdata, scene.kernel_files, scene.steps if steps is None else steps)
+        if handle is not None:
+            self._resident[handle] = (psdata, scene)
+        return handle
+
+    def add(self, scene, steps=None):
+        """Admit `scene` for `steps` steps (default scene.steps).  Returns its
+        handle, or None when the pool has no room for it now."""
+        prepared = self.prepare(scene)
+        try:
+            handle = self.add_prepared(prepared, steps)
+        except Exception:
+            self.discard_prepared(prepared)
+            raise
         if handle is None:
-            free_data_cpu(psdata)
-            return None
-        self._resident[handle] = (psdata, scene)
+            self.discard_prepared(prepared)
         return handle
 
     def step(self, num_steps=1):

@@ -17,6 +17,7 @@ from Python, scripts/sph/systems.py:148-189); the per-scene step sequence is
 the reference's, through the batch (sph.batch) or the single-scene path
 (scenes.build).
 """
+import concurrent.futures
 import dataclasses
 import time
 from collections import deque
@@ -78,18 +79,45 @@ def _summary(name, requests, done, t_start, t_end, extra):
     return rep
 
 
+class _Preparer:
+    """Scene generation and psdata building on worker threads, for requests
+    that have arrived, at most `ahead` prepared and not yet admitted.  The
+    device is only touched by the serving thread."""
+
+    def __init__(self, threads, ahead):
+        from .batch import Batch
+        self._batch_cls = Batch
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=threads)
+        self.futures = {}
+        self.ahead = ahead
+
+    def submit(self, req):
+        if req.index not in self.futures and len(self.futures) < self.ahead:
+            self.futures[req.index] = self.pool.submit(lambda r=req: self._batch_cls.prepare(r.scene()))
+
+    def close(self):
+        for fut in self.futures.values():
+            if not fut.cancel():
+                try:
+                    self._batch_cls.discard_prepared(fut.result())
+                except Exception:
+                    pass
+        self.pool.shutdown(wait=True)
+        self.futures.clear()
+
+
 def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entries=4096, graph_steps=8,
                   compensated_sums=1, polar_fp64=True, realtime=True, on_result=None, fields=("position",),
-                  window=4096, profile=False, log=None):
+                  prepare_threads=2, prepare_ahead=64, profile=False, log=None):
     """Continuous batching over one device pool.
 
     The pool is sized to the HBM budget (or to `slots` / `cells` when given,
-    within the budget).  Each round of the loop admits every arrived request
-    that fits (in arrival order, looking up to `window` requests past ones
-    that do not fit yet), runs
-    `graph_steps` batch steps, then reads back and releases the scenes that
-    finished.  Scene generation for waiting requests overlaps the device's
-    steps.  on_result(request, {field: array}) receives each result."""
+    within the budget).  Worker threads generate the scenes of arrived
+    requests and build their psdata while the device steps.  Each round of
+    the loop admits every prepared request that fits (in arrival order,
+    looking past ones that do not fit yet), runs `graph_steps` batch steps,
+    then reads back and releases the scenes that finished.
+    on_result(request, {field: array}) receives each result."""
     from .batch import Batch
 
     slots = slots or 32 << 20
@@ -100,49 +128,67 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
     pool_bytes = Batch.device_bytes(slots, cells, scene_entries)
     batch = Batch(slots, cells, scene_entries, compensated_sums=compensated_sums, polar_fp64=polar_fp64,
                   graph_steps=0 if profile else graph_steps)
+    prep = _Preparer(prepare_threads, prepare_ahead)
     if profile:
         batch.set_profiling(True)
     stats = {}
     chunk = max(graph_steps, 1)
     pending = deque(sorted(requests, key=lambda r: r.arrival))
-    waiting = []                  # [request, scene or None]
-    running = {}                  # handle -> (request, admitted at)
+    waiting = []                  # arrived, not admitted, in arrival order
+    running = {}                  # handle -> request
     done = {}
     busy_steps = 0
     peak_slots = peak_scenes = 0
-    phase = {"generate": 0.0, "admit": 0.0, "enqueue": 0.0, "wait": 0.0, "read": 0.0, "idle": 0.0}
+    phase = {"prepare_wait": 0.0, "admit": 0.0, "enqueue": 0.0, "wait": 0.0, "read": 0.0, "idle": 0.0}
     t0 = time.perf_counter()
     clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
+
+    def arrivals():
+        now = clock()
+        while pending and pending[0].arrival <= now:
+            waiting.append(pending.popleft())
+        for req in waiting:
+            prep.submit(req)
+
     try:
         while pending or waiting or running:
-            now = clock()
-            while pending and pending[0].arrival <= now:
-                waiting.append([pending.popleft(), None])
+            arrivals()
             # admission: arrival order, looking past requests that do not fit
-            # yet within a window of the queue
+            # yet; stop at the first request still being prepared
             admitted = set()
             misses = 0
-            for k, item in enumerate(waiting[:window]):
-                req, scene = item
-                if scene is None:
-                    tg = time.perf_counter()
-                    scene = item[1] = req.scene()
-                    phase["generate"] += time.perf_counter() - tg
-                if not batch.can_admit(scene):
+            for req in waiting:
+                fut = prep.futures.get(req.index)
+                if fut is None:
+                    break
+                if not fut.done():
+                    if running or admitted:
+                        break
+                    tp = time.perf_counter()
+                    concurrent.futures.wait([fut])       # nothing to step: wait for the scene
+                    phase["prepare_wait"] += time.perf_counter() - tp
+                prepared = fut.result()
+                if not batch.can_admit(prepared[1]):
                     misses += 1
                     if misses >= 4:        # the pool is close to full
                         break
                     continue
                 ta = time.perf_counter()
-                handle = batch.add(scene, req.steps)
+                handle = batch.add_prepared(prepared, req.steps)
                 phase["admit"] += time.perf_counter() - ta
                 if handle is None:
                     continue
-                running[handle] = (req, clock() if realtime else 0.0)
-                admitted.add(k)
-            waiting = [item for k, item in enumerate(waiting) if k not in admitted]
+                del prep.futures[req.index]
+                running[handle] = req
+                admitted.add(req.index)
+            if admitted:
+                waiting = [r for r in waiting if r.index not in admitted]
             if not running:
-                if pending:
+                if waiting and waiting[0].index in prep.futures and prep.futures[waiting[0].index].done():
+                    from .bindings import SphError
+                    raise SphError(f"request {waiting[0].index} does not fit the empty pool "
+                                   f"({slots} slots, {cells} cells)")
+                if pending and not waiting:
                     ti = time.perf_counter()
                     time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.05)))
                     phase["idle"] += time.perf_counter() - ti
@@ -154,22 +200,13 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
             batch.step(chunk)
             busy_steps += chunk
             phase["enqueue"] += time.perf_counter() - te
-            # generate scenes of requests that arrived meanwhile while the
-            # device works
-            now = clock()
-            while pending and pending[0].arrival <= now:
-                waiting.append([pending.popleft(), None])
-            tg = time.perf_counter()
-            for item in waiting[:8]:
-                if item[1] is None:
-                    item[1] = item[0].scene()
-            phase["generate"] += time.perf_counter() - tg
+            arrivals()                     # preparation continues on the workers
             tw = time.perf_counter()
             batch.synchronize()
             phase["wait"] += time.perf_counter() - tw
             tr = time.perf_counter()
             for handle in [h for h in running if batch.steps_left(h) == 0]:
-                req, _ = running.pop(handle)
+                req = running.pop(handle)
                 result = batch.read(handle, *fields)
                 batch.remove(handle)
                 t_done = time.perf_counter() - t0
@@ -185,6 +222,7 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
         if profile:
             stats = batch.kernel_stats()
     finally:
+        prep.close()
         batch.free()
     t_first = requests[0].arrival if realtime else 0.0
     return _summary("batched", requests, done, min(t_first, t_end), t_end,
