@@ -154,9 +154,16 @@ def run_single(args):
     import numpy as np
 
     import sph
-    from sph import roofline, scenes
+    from sph import autotune, roofline, scenes
 
     scene = scenes.baseline_config(WORKLOADS[args.workload])
+    if args.autotune == "run":
+        tuning, _ = autotune.tune_single(scene, log=log)
+    elif args.autotune == "load":
+        tuning = autotune.lookup(autotune.single_signature(scene))
+    else:
+        tuning = dict(autotune.DEFAULTS)
+    autotune.apply(tuning)
     log(f"workload {scene.name}: {scene.n} particles, system {scene.system}")
     system = scenes.build(scene)
     handle = system.psdata_opencl
@@ -221,7 +228,7 @@ def run_single(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded lattice with uniform jitter)",
         "config": {"workload": scene.name, "particles": scene.n, "cells": cells, "system": scene.system,
-                   "path": "plain", "sums": args.sums,
+                   "path": "plain", "sums": args.sums, "tuning": tuning,
                    "fluid_model": "tait eos, mass-consistent force" if scene.system == "fluid" else None},
         "roofline": roof, "cpu_baseline": cpu,
     }
@@ -237,9 +244,20 @@ def _brief(rep):
 
 
 def run_served(args):
-    from sph import roofline, serve
+    from sph import autotune, roofline, serve
 
     steps = args.steps if args.steps_set else 200
+    if args.autotune == "run":
+        # tuned on requests of another seed, not on the measured stream
+        sample = serve.make_requests(count=64, rate=args.rate, seed=args.seed + 2, steps=60)
+        tuning, _ = autotune.tune_batch(sample, log=log)
+    elif args.autotune == "load":
+        tuning = autotune.lookup(autotune.batch_signature())
+    else:
+        tuning = dict(autotune.DEFAULTS)
+    autotune.apply(tuning)
+    if args.graph_steps is None:
+        args.graph_steps = tuning["graph_steps"]
     sums = SUMS[args.sums]
     reqs = serve.make_requests(count=args.requests, rate=args.rate, seed=args.seed, steps=steps)
     prefix = reqs[:args.plain_requests]
@@ -316,7 +334,7 @@ def run_served(args):
         "config": {"workload": "served_stream", "requests": len(reqs), "rate_per_s": args.rate,
                    "mix": "50% fluid / 30% corotational / 20% svk", "particles": "log-uniform 1024-262144",
                    "steps_per_request": steps, "hbm_budget_gb": args.hbm_budget_gb, "sums": args.sums,
-                   "polar": "fp64", "graph_steps": args.graph_steps, "path": "batched"},
+                   "polar": "fp64", "graph_steps": args.graph_steps, "path": "batched", "tuning": tuning},
         "batched_saturated": _brief(sat),
         "prefix": {"requests": len(prefix), "batched": _brief(sat_prefix), "plain": _brief(plain_prefix),
                    "speedup": round(sat_prefix["particle_steps_per_s"] / plain_prefix["particle_steps_per_s"], 3)},
@@ -346,7 +364,10 @@ def main():
     ap.add_argument("--plain-served", type=int, default=1, help="served stream: also serve the prefix one at a time")
     ap.add_argument("--profile-requests", type=int, default=64, help="served stream: prefix of the profiled pass")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0, help="served stream: admission budget")
-    ap.add_argument("--graph-steps", type=int, default=8, help="served stream: batch steps per hipGraph (even)")
+    ap.add_argument("--graph-steps", type=int, default=None,
+                    help="served stream: batch steps per hipGraph (even; default: tuned)")
+    ap.add_argument("--autotune", default="load", choices=["load", "run", "off"],
+                    help="launch parameters: the tuned table (load), tune now (run), or defaults (off)")
     ap.add_argument("--counters-only", action="store_true",
                     help="served stream: run only the profiled pass (for rocprofv3 counter collection)")
     args = ap.parse_args()

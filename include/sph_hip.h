@@ -184,6 +184,17 @@ int get_kernel_stats_hip(psdata_hip *pso, int capacity, const char **names,
  * (3), total kinetic energy, and the centre of mass (3).  out has 7 doubles. */
 int global_sums_hip(psdata_hip *pso, double *out);
 
+/* No counterpart in the reference (its work-group size is fixed at 256,
+ * src/opencl/particle_system_host.c:361-528): threads per block (64, 128 or
+ * 256) of the neighbour passes that accept any of them, process-wide;
+ * chosen by the autotuner (sph.autotune).  Launches captured into a batch's
+ * graph keep the size they were captured with. */
+int sph_hip_set_tuning(unsigned neighbour_block);
+int sph_hip_get_tuning(unsigned *neighbour_block);
+/* Name, architecture and compute-unit count of the selected device (the key
+ * of the autotuner's table). */
+int sph_hip_device_name(char *buf, unsigned len);
+
 /* ---- scene batch --------------------------------------------------------
  * No counterpart in the reference, which runs one psdata per OpenCL program
  * and steps it with one launch per kernel (src/opencl/particle_system_host.c:

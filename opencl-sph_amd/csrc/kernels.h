@@ -72,6 +72,15 @@ struct FieldPtrs {
     const unsigned *celloffset, *gridcount;        // per cell
 };
 
+// ---- launch tuning ------------------------------------------------------------
+// Threads per block of the neighbour passes that take any block size of 64,
+// 128 or 256 (the list consumers and the candidate scans; the
+// LDS-compaction density passes keep their fixed block).  Chosen by the
+// autotuner through sph_hip_set_tuning; 256 by default.
+constexpr unsigned kMaxNeighbourBlock = 256;
+unsigned neighbour_block();
+bool set_neighbour_block(unsigned threads);
+
 // ---- neighbour grid (kernels_grid.hip) ------------------------------------
 struct BinScratch {
     unsigned *key;        // [pnum] cell of each old slot

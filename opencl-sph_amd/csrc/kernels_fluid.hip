@@ -92,7 +92,7 @@ __device__ __forceinline__ float density_scan(const StepParams &p, const FieldPt
 // ORIGINAL = compute_original_density (density0, no neighbour count).
 template <class PS, bool ORIGINAL, bool KAHAN>
 __global__ void __launch_bounds__(kBlock) k_density(PS ps, FieldPtrs f) {
-    const unsigned i = ps.slot_xcd(kBlock);
+    const unsigned i = ps.slot_xcd(blockDim.x);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const V3 xi = load3(f.position, i);
@@ -360,7 +360,7 @@ struct FluidForce {
 
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
-    const unsigned i = ps.slot_xcd(kBlock);
+    const unsigned i = ps.slot_xcd(blockDim.x);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
 // equation of state per pair).  Overflowed lists fall back to the scan.
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtrs f, PairCache c) {
-    const unsigned i = ps.slot_xcd(kBlock);
+    const unsigned i = ps.slot_xcd(blockDim.x);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
@@ -518,7 +518,8 @@ inline dim3 nbr_grid(unsigned n, unsigned block) { return dim3(xcd_grid((n + blo
 template <class PS>
 hipError_t density(const PS &ps, const FieldPtrs &f, bool original, int compensated, hipStream_t s) {
     if (ps.slots() == 0) return hipSuccess;
-    const dim3 g = nbr_grid(ps.slots(), kBlock), b(kBlock);
+    const unsigned nb = neighbour_block();
+    const dim3 g = nbr_grid(ps.slots(), nb), b(nb);
     const bool k = (compensated & kKahanDensity) != 0;
     if (original) {
         if (k) hipLaunchKernelGGL((k_density<PS, true, true>), g, b, 0, s, ps, f);
@@ -552,7 +553,8 @@ hipError_t density_solid_listed(const PS &ps, const FieldPtrs &f, const PairCach
 template <class PS>
 hipError_t forces_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated, hipStream_t s) {
     if (ps.slots() == 0) return hipSuccess;
-    const dim3 g = nbr_grid(ps.slots(), kBlock), b(kBlock);
+    const unsigned nb = neighbour_block();
+    const dim3 g = nbr_grid(ps.slots(), nb), b(nb);
     if (compensated & kKahanForces) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true>), g, b, 0, s, ps, f, c);
     else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false>), g, b, 0, s, ps, f, c);
     return hipGetLastError();
@@ -600,10 +602,11 @@ hipError_t launch_compute_original_density(const StepParams &p, const FieldPtrs 
 hipError_t launch_forces_fluids(const StepParams &p, const FieldPtrs &f, hipStream_t s) {
     if (p.n == 0) return hipSuccess;
     const SingleScene ps{p};
+    const unsigned nb = neighbour_block();
     if (p.compensated & kKahanForces)
-        hipLaunchKernelGGL((k_forces_fluids<SingleScene, true>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, ps, f);
+        hipLaunchKernelGGL((k_forces_fluids<SingleScene, true>), nbr_grid(p.n, nb), dim3(nb), 0, s, ps, f);
     else
-        hipLaunchKernelGGL((k_forces_fluids<SingleScene, false>), nbr_grid(p.n, kBlock), dim3(kBlock), 0, s, ps, f);
+        hipLaunchKernelGGL((k_forces_fluids<SingleScene, false>), nbr_grid(p.n, nb), dim3(nb), 0, s, ps, f);
     return hipGetLastError();
 }
 

@@ -133,7 +133,7 @@ __device__ __forceinline__ bool in_body(const FieldPtrs &f, unsigned i) {
 
 template <class PS, bool KAHAN, bool FP64, int MODE>
 __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f, PairCache c) {
-    const unsigned i = ps.slot_xcd(kBlock);
+    const unsigned i = ps.slot_xcd(blockDim.x);
     if (!ps.active(i) || in_body(f, i)) return;
     const unsigned cnt = solid_list_count<MODE>(c, i);
     if (skip_lane<MODE>(cnt)) return;
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(kBlock) k_stresses_corot(PS ps, FieldPtrs f) {
 
 template <class PS, bool KAHAN, bool FP64, int MODE>
 __global__ void __launch_bounds__(kBlock) k_deformations_svk(PS ps, FieldPtrs f, PairCache c) {
-    const unsigned i = ps.slot_xcd(kBlock);
+    const unsigned i = ps.slot_xcd(blockDim.x);
     if (!ps.active(i)) return;
     const unsigned cnt = solid_list_count<MODE>(c, i);
     if (skip_lane<MODE>(cnt)) return;
@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(kBlock) k_stresses_svk(PS ps, FieldPtrs f) {
 
 template <class PS, bool CORO, bool KAHAN, int MODE>
 __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, PairCache c) {
-    const unsigned i = ps.slot_xcd(kBlock);
+    const unsigned i = ps.slot_xcd(blockDim.x);
     if (!ps.active(i)) return;
     const unsigned cnt = solid_list_count<MODE>(c, i);
     if (skip_lane<MODE>(cnt)) return;
@@ -471,14 +471,15 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, Pa
 }
 
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
-inline dim3 nbr_grid(unsigned n) { return dim3(xcd_grid((n + kBlock - 1) / kBlock)); }
+inline dim3 nbr_grid(unsigned n, unsigned block) { return dim3(xcd_grid((n + block - 1) / block)); }
 
 // Launches kernel template K<MODE> as the list-only and overflow-scan pair
 // when `c` holds a solid list, else as one scan over every lane.
 template <template <int> class K, class PS>
 hipError_t launch_modes(const PS &ps, const FieldPtrs &f, const PairCache &c, hipStream_t s) {
     if (ps.slots() == 0) return hipSuccess;
-    const dim3 g = nbr_grid(ps.slots()), b(kBlock);
+    const unsigned nb = neighbour_block();
+    const dim3 g = nbr_grid(ps.slots(), nb), b(nb);
     if (c.valid == kSolidList) {
         hipLaunchKernelGGL(K<kListedOnly>::template kernel<PS>, g, b, 0, s, ps, f, c);
         hipLaunchKernelGGL(K<kScanOverflow>::template kernel<PS>, g, b, 0, s, ps, f, c);

@@ -600,6 +600,45 @@ bool same_psdata(const psdata &a, const psdata &b) { return a.data == b.data && 
 }  // namespace
 
 // ============================================================================
+// Launch tuning
+// ============================================================================
+namespace sphk {
+namespace {
+unsigned g_neighbour_block = kMaxNeighbourBlock;
+}
+unsigned neighbour_block() { return g_neighbour_block; }
+bool set_neighbour_block(unsigned threads) {
+    if (threads != 64 && threads != 128 && threads != 256) return false;
+    g_neighbour_block = threads;
+    return true;
+}
+}  // namespace sphk
+
+extern "C" int sph_hip_set_tuning(unsigned neighbour_block) {
+    if (!sphk::set_neighbour_block(neighbour_block))
+        return fail("neighbour block size must be 64, 128 or 256 (got %u)", neighbour_block);
+    return 0;
+}
+
+extern "C" int sph_hip_device_name(char *buf, unsigned len) {
+    if (!buf || len == 0) return fail("null argument");
+    int dev = 0;
+    if (sphhip::current_device(&dev)) return -1;
+    hipDeviceProp_t prop;
+    SPH_HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    std::string arch(prop.gcnArchName);
+    arch = arch.substr(0, arch.find(':'));   // drop the target features
+    std::snprintf(buf, len, "%s %dCU", arch.c_str(), prop.multiProcessorCount);
+    return 0;
+}
+
+extern "C" int sph_hip_get_tuning(unsigned *neighbour_block) {
+    if (!neighbour_block) return fail("null argument");
+    *neighbour_block = sphk::neighbour_block();
+    return 0;
+}
+
+// ============================================================================
 // Helpers shared with the scene batch
 // ============================================================================
 namespace sphhip {

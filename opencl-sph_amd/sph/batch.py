@@ -30,7 +30,7 @@ def fill_fields(psdata, values):
         if isinstance(value, np.ndarray) and value.size > 1:
             np.copyto(field[tuple(slice(None, s) for s in value.shape)], value)
         else:
-            np.copyto(field, value)
+            field[...] = value
 
 
 class Batch:

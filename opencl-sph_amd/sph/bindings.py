@@ -133,8 +133,29 @@ def _names_array(names):
     return arr
 
 
+_set_tuning = _fn("sph_hip_set_tuning", [c_uint], c_int)
+_get_tuning = _fn("sph_hip_get_tuning", [POINTER(c_uint)], c_int)
+_device_name = _fn("sph_hip_device_name", [ctypes.c_char_p, c_uint], c_int)
+
+
 def init_hip():
     check(_init_hip())
+
+
+def set_tuning(neighbour_block):
+    check(_set_tuning(int(neighbour_block)))
+
+
+def get_tuning():
+    out = c_uint()
+    check(_get_tuning(ctypes.byref(out)))
+    return {"neighbour_block": out.value}
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    check(_device_name(buf, 256))
+    return buf.value.decode()
 
 
 def terminate_hip():

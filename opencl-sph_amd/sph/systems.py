@@ -81,7 +81,7 @@ class ParticleSystem(object):
             if isinstance(value, np.ndarray) and value.size > 1:
                 np.copyto(field[tuple(slice(None, s) for s in value.shape)], value)
             else:
-                np.copyto(field, value)
+                field[...] = value          # scalars broadcast, cast to the field's type
         if not host_only:
             self.sync_fields_host_to_device(*names)
 
@@ -153,7 +153,7 @@ class StrainSolid(ParticleSystem):
     def init(self, **kwargs):
         super(StrainSolid, self).init(**kwargs)
         if self.init_values is None:
-            self.init_values = {k: np.copy(v) for k, v in kwargs.items()}
+            self.init_values = {k: (v if np.isscalar(v) else np.copy(v)) for k, v in kwargs.items()}
 
     def setup(self):
         """Reference configuration (scripts/sph/systems.py:139-146)."""

@@ -91,3 +91,22 @@ def test_twenty_steps_drift(system):
     assert np.allclose(sums[0], mom, rtol=0, atol=1e-12 * max(scale, 1e-30))
     assert abs(sums[1] - ke) <= 1e-12 * abs(ke)
     assert np.allclose(sums[2], com, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("system", ["fluid", "corotational", "svk"])
+def test_tuned_block_sizes_change_no_bit(system):
+    """Every neighbour block size the autotuner may pick gives the same bits."""
+    from sph import bindings
+    sc = small(system)
+    fields = ("position", "velocity", "density", "force")
+    before = bindings.get_tuning()["neighbour_block"]
+    try:
+        outs = {}
+        for nb in (64, 128, 256):
+            bindings.set_tuning(nb)
+            outs[nb] = run(sc, 5, 1, True, fields)[0]
+    finally:
+        bindings.set_tuning(before)
+    for nb in (64, 128):
+        for k in fields:
+            assert np.array_equal(outs[nb][k], outs[256][k]), (system, nb, k)
