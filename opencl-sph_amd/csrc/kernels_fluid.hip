@@ -20,6 +20,9 @@ namespace sphk {
 namespace {
 
 constexpr unsigned kBlock = 256;
+constexpr unsigned kPairGroup = 8;   // list entries whose loads are issued together
+constexpr unsigned kScanUnroll = 8;  // candidates whose loads are issued together in a scan
+constexpr unsigned kSolidScanUnroll = 4;   // the same for the solid scan (two positions per candidate)
 
 template <bool KAHAN>
 struct Acc {
@@ -167,12 +170,21 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
             unsigned b, e;
             if (!row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e)) continue;
             if (e - b > (1u << kRowBits)) fast = false;
-            for (unsigned j = b; j < e; ++j) {
-                const float s = sqlength(load3(pos, j) - xi);
-                if (!(s >= r2_out)) {
-                    if (cnt < kListCap)
-                        col[cnt * kListBlock] = static_cast<unsigned short>((r << kRowBits) | (j - b));
-                    ++cnt;
+            // kScanUnroll candidates per round, loads first (the last valid
+            // slot stands in past the end), tested in slot order
+            for (unsigned j0 = b; j0 < e; j0 += kScanUnroll) {
+                V3 xs[kScanUnroll];
+#pragma unroll
+                for (unsigned u = 0; u < kScanUnroll; ++u) xs[u] = load3(pos, min(j0 + u, e - 1u));
+#pragma unroll
+                for (unsigned u = 0; u < kScanUnroll; ++u) {
+                    if (j0 + u >= e) break;
+                    const float s = sqlength(xs[u] - xi);
+                    if (!(s >= r2_out)) {
+                        if (cnt < kListCap)
+                            col[cnt * kListBlock] = static_cast<unsigned short>((r << kRowBits) | (j0 + u - b));
+                        ++cnt;
+                    }
                 }
             }
         }
@@ -183,18 +195,32 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     if (fast) {
         Acc<KAHAN> acc;
         unsigned cur = 0xffu, base = 0, e = 0;
-        for (unsigned k = 0; k < cnt; ++k) {
-            const unsigned ent = col[k * kListBlock];
-            const unsigned r = ent >> kRowBits;
-            if (r != cur) {
-                cur = r;
-                row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &base, &e);
+        for (unsigned k0 = 0; k0 < cnt; k0 += kPairGroup) {
+            unsigned js[kPairGroup];
+#pragma unroll
+            for (unsigned q = 0; q < kPairGroup; ++q) {
+                js[q] = i;
+                if (k0 + q < cnt) {
+                    const unsigned ent = col[(k0 + q) * kListBlock];
+                    const unsigned r = ent >> kRowBits;
+                    if (r != cur) {
+                        cur = r;
+                        row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &base, &e);
+                    }
+                    js[q] = base + (ent & ((1u << kRowBits) - 1u));
+                }
             }
-            const unsigned j = base + (ent & ((1u << kRowBits) - 1u));
-            const float rr = length(load3(pos, j) - xi);
-            acc.add(w_poly6(rr, kc));
-            if (rr < kc.h) ++nn;
-            c.list[static_cast<size_t>(k) * c.stride + i] = j;
+            V3 xs[kPairGroup];
+#pragma unroll
+            for (unsigned q = 0; q < kPairGroup; ++q) xs[q] = load3(pos, js[q]);
+#pragma unroll
+            for (unsigned q = 0; q < kPairGroup; ++q) {
+                if (k0 + q >= cnt) break;
+                const float rr = length(xs[q] - xi);
+                acc.add(w_poly6(rr, kc));
+                if (rr < kc.h) ++nn;
+                c.list[static_cast<size_t>(k0 + q) * c.stride + i] = js[q];
+            }
         }
         rho = acc.value();
     } else {
@@ -254,14 +280,25 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                 unsigned b, e;
                 if (!row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e)) continue;
                 if (e - b > (1u << kSolidOffsetBits)) fast = false;
-                for (unsigned j = b; j < e; ++j) {
-                    const bool cur = !(sqlength(load3(pos, j) - xi) >= r2_out);
-                    const bool ref = !(sqlength(load3(opos, j) - x0i) >= r2_out);
-                    if (cur || ref) {
-                        if (cnt < kListCap)
-                            col[cnt * kListBlock] = static_cast<unsigned short>(
-                                (r << kRowBits) | (cur ? 1u << 11 : 0u) | (ref ? 1u << 10 : 0u) | (j - b));
-                        ++cnt;
+                for (unsigned j0 = b; j0 < e; j0 += kSolidScanUnroll) {
+                    V3 xs[kSolidScanUnroll], x0s[kSolidScanUnroll];
+#pragma unroll
+                    for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
+                        const unsigned j = min(j0 + u, e - 1u);
+                        xs[u] = load3(pos, j);
+                        x0s[u] = load3(opos, j);
+                    }
+#pragma unroll
+                    for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
+                        if (j0 + u >= e) break;
+                        const bool cur = !(sqlength(xs[u] - xi) >= r2_out);
+                        const bool ref = !(sqlength(x0s[u] - x0i) >= r2_out);
+                        if (cur || ref) {
+                            if (cnt < kListCap)
+                                col[cnt * kListBlock] = static_cast<unsigned short>(
+                                    (r << kRowBits) | (cur ? 1u << 11 : 0u) | (ref ? 1u << 10 : 0u) | (j0 + u - b));
+                            ++cnt;
+                        }
                     }
                 }
             }
@@ -270,22 +307,37 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
         Acc<KAHAN> acc;
         if (fast) {
             unsigned cur_row = 0xffu, base = 0, e = 0;
-            for (unsigned k = 0; k < cnt; ++k) {
-                const unsigned ent = col[k * kListBlock];
-                const unsigned r = ent >> kRowBits;
-                if (r != cur_row) {
-                    cur_row = r;
-                    row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &base, &e);
+            for (unsigned k0 = 0; k0 < cnt; k0 += kPairGroup) {
+                unsigned ents[kPairGroup], js[kPairGroup];
+#pragma unroll
+                for (unsigned q = 0; q < kPairGroup; ++q) {
+                    ents[q] = 0;
+                    js[q] = i;
+                    if (k0 + q < cnt) {
+                        ents[q] = col[(k0 + q) * kListBlock];
+                        const unsigned r = ents[q] >> kRowBits;
+                        if (r != cur_row) {
+                            cur_row = r;
+                            row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &base, &e);
+                        }
+                        js[q] = base + (ents[q] & ((1u << kSolidOffsetBits) - 1u));
+                    }
                 }
-                const unsigned j = base + (ent & ((1u << kSolidOffsetBits) - 1u));
-                const bool cur = (ent >> 11) & 1u;
-                if (cur) {
-                    const float rr = length(load3(pos, j) - xi);
-                    acc.add(w_poly6(rr, kc));
-                    if (rr < kc.h) ++nn;
+                V3 xs[kPairGroup];
+#pragma unroll
+                for (unsigned q = 0; q < kPairGroup; ++q) xs[q] = load3(pos, js[q]);
+#pragma unroll
+                for (unsigned q = 0; q < kPairGroup; ++q) {
+                    if (k0 + q >= cnt) break;
+                    const bool cur = (ents[q] >> 11) & 1u;
+                    if (cur) {
+                        const float rr = length(xs[q] - xi);
+                        acc.add(w_poly6(rr, kc));
+                        if (rr < kc.h) ++nn;
+                    }
+                    c.list[static_cast<size_t>(k0 + q) * c.stride + i] =
+                        js[q] | (cur ? kListCurrent : 0u) | (((ents[q] >> 10) & 1u) ? kListReference : 0u);
                 }
-                c.list[static_cast<size_t>(k) * c.stride + i] =
-                    j | (cur ? kListCurrent : 0u) | (((ent >> 10) & 1u) ? kListReference : 0u);
             }
         } else {
             cnt = 0;
@@ -404,12 +456,15 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
     if (cnt <= kListCap) {
         // List entries are in range by construction (same positions, same
         // distance expression), so the pair terms need no range test here.
-        unsigned j = cnt ? c.list[i] : 0u;
-        for (unsigned k = 0; k < cnt; ++k) {
-            const unsigned jn = k + 1 < cnt ? c.list[static_cast<size_t>(k + 1) * c.stride + i] : 0u;
-            if (j != i) acc.pair(visc, kc, xi, vi, rhoi, pi, load3(pos, j), load3(vel, j), rho[j], prs[j]);
-            j = jn;
-        }
+        struct Nbr {
+            V3 x, v;
+            float rho, p;
+        };
+        for_each_listed<kPairGroup>(
+            c, i, cnt, [&](unsigned j) { return Nbr{load3(pos, j), load3(vel, j), rho[j], prs[j]}; },
+            [&](unsigned j, const Nbr &n) {
+                if (j != i) acc.pair(visc, kc, xi, vi, rhoi, pi, n.x, n.v, n.rho, n.p);
+            });
     } else {
         for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
             if (j == i) return;

@@ -19,6 +19,10 @@ namespace sphk {
 namespace {
 
 constexpr unsigned kBlock = 256;
+// List entries whose neighbour loads are issued together: the rotation and
+// deformation sweeps load two positions per entry, the force pass ~27 floats.
+constexpr unsigned kSweepGroup = 8;
+constexpr unsigned kForceGroup = 4;
 
 template <bool KAHAN>
 struct Acc {
@@ -147,16 +151,19 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
     const unsigned key = f.gridcell[i];
 
     Acc<KAHAN> apq[9];
+    struct Pos2 {
+        V3 x, x0;
+    };
+    auto load_pos2 = [&](unsigned e) {
+        const unsigned j = e & kListSlot;
+        return Pos2{load3(pos, j), load3(opos, j)};
+    };
     if (MODE == kListedOnly) {
-        unsigned next = cnt ? list_entry(c, 0, i) : 0u;
-        for (unsigned k = 0; k < cnt; ++k) {
-            const unsigned e = next;
-            const unsigned j = e & kListSlot;
-            next = k + 1 < cnt ? list_entry(c, k + 1, i) : 0u;
-            if (!(e & kListCurrent) || j == i) continue;
-            const V3 pv = load3(pos, j) - xi;
-            add_apq(apq, pv, load3(opos, j) - x0i, w_poly6(length(pv), kc));
-        }
+        for_each_listed<kSweepGroup>(c, i, cnt, load_pos2, [&](unsigned e, const Pos2 &d) {
+            if (!(e & kListCurrent) || (e & kListSlot) == i) return;
+            const V3 pv = d.x - xi;
+            add_apq(apq, pv, d.x0 - x0i, w_poly6(length(pv), kc));
+        });
     } else for_each_candidate(p.grid, p.p0, p.p0 + p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
@@ -192,16 +199,12 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
     const float mass = p.mass;   // read once (see k_forces_solids)
     Acc<KAHAN> du[9];
     if (MODE == kListedOnly) {
-        unsigned next = cnt ? list_entry(c, 0, i) : 0u;
-        for (unsigned k = 0; k < cnt; ++k) {
-            const unsigned e = next;
-            const unsigned j = e & kListSlot;
-            next = k + 1 < cnt ? list_entry(c, k + 1, i) : 0u;
-            if (!(e & kListReference) || j == i) continue;
-            const V3 q = load3(opos, j) - x0i;
-            const V3 u = mat_mul(rt, load3(pos, j) - xi) - q;
+        for_each_listed<kSweepGroup>(c, i, cnt, load_pos2, [&](unsigned e, const Pos2 &d) {
+            if (!(e & kListReference) || (e & kListSlot) == i) return;
+            const V3 q = d.x0 - x0i;
+            const V3 u = mat_mul(rt, d.x - xi) - q;
             add_outer(du, (u * mass) / d0i, grad_spiky(-q, kc));
-        }
+        });
     } else for_each_candidate(p.grid, p.p0, p.p0 + p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
@@ -270,18 +273,23 @@ __global__ void __launch_bounds__(kBlock) k_deformations_svk(PS ps, FieldPtrs f,
     const unsigned ncombo = 1u << p.nplanes;
     Acc<KAHAN> apq[9], aqq[6];
     if (MODE == kListedOnly) {
-        unsigned next = cnt ? list_entry(c, 0, i) : 0u;
-        for (unsigned k = 0; k < cnt; ++k) {
-            const unsigned e = next;
-            const unsigned j = e & kListSlot;
-            next = k + 1 < cnt ? list_entry(c, k + 1, i) : 0u;
-            if (!(e & kListCurrent) || j == i) continue;
-            const V3 pv = load3(pos, j) - xi;
-            const V3 q = load3(opos, j) - x0i;
-            const float w = w_poly6(length(pv), kc);
-            add_apq(apq, pv, q, w);
-            add_aqq(aqq, q, w);
-        }
+        struct Pos2 {
+            V3 x, x0;
+        };
+        for_each_listed<kSweepGroup>(
+            c, i, cnt,
+            [&](unsigned e) {
+                const unsigned j = e & kListSlot;
+                return Pos2{load3(pos, j), load3(opos, j)};
+            },
+            [&](unsigned e, const Pos2 &d) {
+                if (!(e & kListCurrent) || (e & kListSlot) == i) return;
+                const V3 pv = d.x - xi;
+                const V3 q = d.x0 - x0i;
+                const float w = w_poly6(length(pv), kc);
+                add_apq(apq, pv, q, w);
+                add_aqq(aqq, q, w);
+            });
     } else for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
@@ -402,19 +410,31 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, Pa
         fvc.add(v);
     };
     if (MODE == kListedOnly) {
-        unsigned next = cnt ? list_entry(c, 0, i) : 0u;
-        for (unsigned k = 0; k < cnt; ++k) {
-            const unsigned j = next & kListSlot;
-            next = k + 1 < cnt ? list_entry(c, k + 1, i) : 0u;
-            if (j == i) continue;
-            const float vivj = mass * mass / d0i / f.density0[j];
-            float sj[6], rj[9];
-            for (int m = 0; m < 6; ++m) sj[m] = stress[6 * j + m];
-            if (CORO)
-                for (int m = 0; m < 9; ++m) rj[m] = rot[9 * j + m];
-            const float mrho = mass / f.density[j];
-            pair(j, load3(pos, j), load3(opos, j), load3(vel, j), vivj, sj, rj, mrho);
-        }
+        struct Nbr {
+            V3 x, x0, v;
+            float d0, rho, s[6], r[CORO ? 9 : 1];
+        };
+        for_each_listed<kForceGroup>(
+            c, i, cnt,
+            [&](unsigned e) {
+                const unsigned j = e & kListSlot;
+                Nbr n;
+                n.x = load3(pos, j);
+                n.x0 = load3(opos, j);
+                n.v = load3(vel, j);
+                n.d0 = f.density0[j];
+                n.rho = f.density[j];
+                for (int m = 0; m < 6; ++m) n.s[m] = stress[6 * j + m];
+                if (CORO)
+                    for (int m = 0; m < 9; ++m) n.r[m] = rot[9 * j + m];
+                return n;
+            },
+            [&](unsigned e, const Nbr &n) {
+                const unsigned j = e & kListSlot;
+                if (j == i) return;
+                const float vivj = mass * mass / d0i / n.d0;
+                pair(j, n.x, n.x0, n.v, vivj, n.s, CORO ? n.r : nullptr, mass / n.rho);
+            });
     } else for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 x0j = load3(opos, j);
         const V3 xj = load3(pos, j);

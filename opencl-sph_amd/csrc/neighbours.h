@@ -46,6 +46,29 @@ __device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned l
     }
 }
 
+// Iterates slot i's pair-cache list in groups of G entries: the G entries
+// are loaded, then load(entry) is issued for all G of them, then
+// use(entry, data) runs on each in list order.  Issuing a group's loads
+// together lets their memory latency overlap; the order in which use()
+// sees the entries, and so every sum, is the list order.  Entries past the
+// end read slot i's own data (load must accept it) and are not used.
+template <unsigned G, typename Load, typename Use>
+__device__ __forceinline__ void for_each_listed(const PairCache &c, unsigned i, unsigned cnt, Load &&load,
+                                                Use &&use) {
+    for (unsigned k0 = 0; k0 < cnt; k0 += G) {
+        unsigned e[G];
+#pragma unroll
+        for (unsigned g = 0; g < G; ++g)
+            e[g] = k0 + g < cnt ? c.list[static_cast<size_t>(k0 + g) * c.stride + i] : i;
+        decltype(load(0u)) d[G];
+#pragma unroll
+        for (unsigned g = 0; g < G; ++g) d[g] = load(e[g]);
+#pragma unroll
+        for (unsigned g = 0; g < G; ++g)
+            if (k0 + g < cnt) use(e[g], d[g]);
+    }
+}
+
 __device__ __forceinline__ V3 plane_point(const StepParams &p, unsigned s) {
     return V3{p.planes[s][0], p.planes[s][1], p.planes[s][2]};
 }
