@@ -162,6 +162,30 @@ int step_forward_device_hip(psdata_hip *pso);
 int step_device_hip(psdata_hip *pso, const char *system, unsigned int num_steps,
                     int apply_constraints);
 
+/* Initial-condition helpers (particle_system_host.c:530-557,590-608; kernels
+ * populate_position_cuboid and rotate_particles, kernels/common.cl:533-604).
+ * populate: particle i + xsize*j + xsize*ysize*k (below pnum) is placed at
+ * corner1 + (index / (size - 1)) * (corner2 - corner1) per axis; n becomes
+ * pnum when the lattice reaches it, else the lattice size.  rotate: every
+ * active position is multiplied by Rx(angle_x) Ry(angle_y) Rz(angle_z) with
+ * the reference's axis factors (its Ry carries -sin in row 0). */
+int populate_position_cuboid_device_hip(psdata_hip *pso, float x1, float y1, float z1,
+                                        float x2, float y2, float z2, unsigned int xsize,
+                                        unsigned int ysize, unsigned int zsize);
+int rotate_particles_device_hip(psdata_hip *pso, float angle_x, float angle_y, float angle_z);
+
+/* Scripted rigid bodies (particle_system_host.c:562-588; kernels/rigidbody.cl,
+ * which must be in the kernel file list as 'rigidbody').  Particles with
+ * body_number == body + 1 (body below body_count) follow the body:
+ * set_body_position moves them by position - body_position[body] and gives
+ * them that displacement / timestep as velocity; set_body_rotation applies
+ * B^T P (B: matrix of the stored Euler angles, P: the particle's rotation
+ * field) to position and rotation.  Then the body's stored position / angles
+ * become the arguments (float[3]).  A body outside body_position /
+ * body_euler_rotation is an error (the reference writes past the array). */
+int set_body_position_device_hip(psdata_hip *pso, unsigned int body, const float *position);
+int set_body_rotation_device_hip(psdata_hip *pso, unsigned int body, const float *euler_angles);
+
 /* Block until all work queued on the handle's stream has finished. */
 int synchronize_hip(psdata_hip *pso);
 

@@ -24,6 +24,7 @@ SOURCES = [
     "kernels_grid.hip",
     "kernels_fluid.hip",
     "kernels_solid.hip",
+    "kernels_bodies.hip",
     "runtime.hip",
     "batch.hip",
 ]

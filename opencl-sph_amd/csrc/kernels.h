@@ -226,4 +226,28 @@ constexpr unsigned kReduceBlocks = 256;
 hipError_t launch_global_sums(const StepParams &p, const FieldPtrs &f, double *partials,
                               double *out7, hipStream_t s);
 
+// Scripted rigid bodies and initial-condition helpers (kernels_bodies.hip).
+// Row-major 3x3 matrix passed by value; Box = two corners.
+struct Mat3 {
+    float m[9];
+};
+struct Box {
+    float lo[3], hi[3];
+};
+// Lattice of sx*sy*sz points into particles [0, count) (slot = particle).
+hipError_t launch_populate_cuboid(float *pos, unsigned count, unsigned sx, unsigned sy, unsigned sz, const Box &c,
+                                  hipStream_t s);
+// position <- m position for slots [0, n).
+hipError_t launch_rotate(float *pos, unsigned n, const Mat3 &m, hipStream_t s);
+// Moves the particles of `body` (when `move`), then stores `target` as the
+// body's position.
+hipError_t launch_set_body_position(float *pos, float *vel, const unsigned *body_number, unsigned n, unsigned body,
+                                    bool move, float *body_position, const float *target, float dt,
+                                    hipStream_t s);
+// Rotates the particles of `body` by the transposed body matrix `bt` (when
+// `move`), then stores `angles` as the body's Euler angles.
+hipError_t launch_set_body_rotation(float *pos, float *rot, const unsigned *body_number, unsigned n, unsigned body,
+                                    bool move, const Mat3 &bt, float *body_euler, const float *angles,
+                                    hipStream_t s);
+
 }  // namespace sphk

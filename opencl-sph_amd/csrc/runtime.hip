@@ -419,7 +419,7 @@ int resolve_kernel(psdata_hip *h, const char *name, std::string *internal) {
     }
     if (n == "set_body_position" || n == "set_body_rotation" || n == "populate_position_cuboid" ||
         n == "rotate_particles")
-        return fail("kernel %s (scripted bodies / initial-condition helpers) is not provided by this engine", name);
+        return fail("kernel %s takes arguments: call %s_device_hip", name, name);
     if (n == "zero_gridcount" || n == "count_particles_in_bins" || n == "prefix_sum" ||
         n == "copy_celloffset_to_backup" || n == "insert_particles_in_bin_array")
         return fail("kernel %s is one stage of compute_particle_bins; call that instead", name);
@@ -502,11 +502,13 @@ bool hot_field(const std::string &name, const KernelSet &k) {
                                         "force_elastic", "force_viscous", "cutoff_force"};
     static const char *const svk[] = {"deformation", "stress", "force_elastic", "force_viscous", "cutoff_force"};
     static const char *const cons[] = {"plane_constraints_particles"};
+    static const char *const rigid[] = {"rotation"};
     for (const char *c : common) if (name == c) return true;
     if (k.solids || k.corot || k.svk) for (const char *c : solids) if (name == c) return true;
     if (k.corot) for (const char *c : corot) if (name == c) return true;
     if (k.svk) for (const char *c : svk) if (name == c) return true;
     if (k.constraints) for (const char *c : cons) if (name == c) return true;
+    if (k.rigidbody) for (const char *c : rigid) if (name == c) return true;
     return false;
 }
 
@@ -1130,6 +1132,184 @@ extern "C" int get_kernel_stats_hip(psdata_hip *h, int capacity, const char **na
         ++k;
     }
     return k;
+}
+
+// ============================================================================
+// Scripted rigid bodies and initial-condition helpers
+// (src/opencl/particle_system_host.c:530-614)
+// ============================================================================
+namespace {
+
+// Products in the reference's order (multiplyMatrices, kernels/common.cl:208-220):
+// o[r][c] = (a[r][0] b[0][c] + a[r][1] b[1][c]) + a[r][2] b[2][c].
+sphk::Mat3 mat_mul(const sphk::Mat3 &a, const sphk::Mat3 &b) {
+    sphk::Mat3 o;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            o.m[3 * r + c] = a.m[3 * r] * b.m[c] + a.m[3 * r + 1] * b.m[3 + c] + a.m[3 * r + 2] * b.m[6 + c];
+    return o;
+}
+
+// The reference's axis factors (kernels/common.cl:575-587, kernels/rigidbody.cl:47-57);
+// its y factor carries -sin in row 0 and +sin in row 2.
+sphk::Mat3 axis_x(float a) {
+    const float c = std::cos(a), s = std::sin(a);
+    return sphk::Mat3{{1.0f, 0.0f, 0.0f, 0.0f, c, -s, 0.0f, s, c}};
+}
+sphk::Mat3 axis_y(float a) {
+    const float c = std::cos(a), s = std::sin(a);
+    return sphk::Mat3{{c, 0.0f, -s, 0.0f, 1.0f, 0.0f, s, 0.0f, c}};
+}
+sphk::Mat3 axis_z(float a) {
+    const float c = std::cos(a), s = std::sin(a);
+    return sphk::Mat3{{c, -s, 0.0f, s, c, 0.0f, 0.0f, 0.0f, 1.0f}};
+}
+
+// Body matrix of Euler angles (x, y, x2) = Rx(x2) (Ry(y) Rx(x)) (get_matrix_from_euler).
+sphk::Mat3 euler_matrix(const float e[3]) { return mat_mul(axis_x(e[2]), mat_mul(axis_y(e[1]), axis_x(e[0]))); }
+
+sphk::Mat3 transposed(const sphk::Mat3 &a) {
+    sphk::Mat3 t;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) t.m[3 * r + c] = a.m[3 * c + r];
+    return t;
+}
+
+// A per-particle field kept in slot order with `words` 32-bit values per particle.
+DevField *particle_field(psdata_hip *h, const char *name, unsigned words, const char *who) {
+    DevField *f = h->field(name);
+    if (!f || f->kind != FieldKind::Particle || !f->hot || f->words != words) {
+        fail("%s needs a per-particle field '%s' with %u values per particle", who, name, words);
+        return nullptr;
+    }
+    return f;
+}
+
+// A per-body float[3] field holding at least body + 1 entries.
+DevField *body_field(psdata_hip *h, const char *name, unsigned body, const char *who) {
+    DevField *f = h->field(name);
+    if (!f || h->host.entry_sizes[f->index] != 4) {
+        fail("%s needs a float field '%s' (3 values per body)", who, name);
+        return nullptr;
+    }
+    if (static_cast<size_t>(body + 1) * 12 > f->bytes) {
+        fail("%s: body %u is outside '%s' (%zu bodies)", who, body, name, f->bytes / 12);
+        return nullptr;
+    }
+    return f;
+}
+
+int body_count_of(psdata_hip *h, const char *who, unsigned *out) {
+    DevField *f = h->field("body_count");
+    if (!f || f->bytes < 4 || h->host.entry_sizes[f->index] != 4)
+        return fail("%s needs an unsigned field 'body_count'", who);
+    std::memcpy(out, h->shadow.data() + h->host.data_offsets[f->index], 4);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int populate_position_cuboid_device_hip(psdata_hip *h, float x1, float y1, float z1, float x2, float y2,
+                                                   float z2, unsigned xsize, unsigned ysize, unsigned zsize) {
+    static const char *who = "populate_position_cuboid";
+    if (!h) return fail("null handle");
+    DevField *pos = particle_field(h, "position", 3, who);
+    if (!pos) return -1;
+    DevField *nf = h->field("n");
+    const unsigned long long points = static_cast<unsigned long long>(xsize) * ysize * zsize;
+    if (points > 0xffffffffull)
+        return fail("%s: %llu lattice points; particle indices are 32-bit", who, points);
+    if (points == 0 || h->pnum == 0) return 0;   // no lane reaches a particle
+    // Lanes at or past pnum write nothing; the lane of particle pnum - 1 sets
+    // n = pnum, else the last lattice point sets n = its count.
+    const unsigned count = static_cast<unsigned>(std::min<unsigned long long>(points, h->pnum));
+    const unsigned new_n = points >= h->pnum ? h->pnum : static_cast<unsigned>(points);
+    // The lattice addresses particles by index: return to particle order.
+    if (reset_permutation(h)) return -1;
+    h->pairs.valid = 0;
+    if (new_n != h->n) {
+        h->n = new_n;
+        char *shadow_n = h->shadow.data() + h->host.data_offsets[nf->index];
+        std::memcpy(shadow_n, &new_n, 4);
+        HIP_TRY(hipMemcpyAsync(nf->ptr(), shadow_n, 4, hipMemcpyHostToDevice, h->stream));
+    }
+    const sphk::Box box{{x1, y1, z1}, {x2, y2, z2}};
+    if (timed(h, who, [&] {
+            return sphk::launch_populate_cuboid(static_cast<float *>(pos->ptr()), count, xsize, ysize, zsize, box,
+                                                h->stream);
+        }))
+        return -1;
+    return 0;
+}
+
+extern "C" int rotate_particles_device_hip(psdata_hip *h, float angle_x, float angle_y, float angle_z) {
+    static const char *who = "rotate_particles";
+    if (!h) return fail("null handle");
+    DevField *pos = particle_field(h, "position", 3, who);
+    if (!pos || ensure_current(h, pos)) return -1;
+    h->pairs.valid = 0;
+    // Rx (Ry Rz), kernels/common.cl:589-592
+    const sphk::Mat3 m = mat_mul(axis_x(angle_x), mat_mul(axis_y(angle_y), axis_z(angle_z)));
+    if (timed(h, who, [&] {
+            return sphk::launch_rotate(static_cast<float *>(pos->ptr()), h->n, m, h->stream);
+        }))
+        return -1;
+    return 0;
+}
+
+extern "C" int set_body_position_device_hip(psdata_hip *h, unsigned body, const float *position) {
+    static const char *who = "set_body_position";
+    if (!h || !position) return fail("null argument");
+    if (!h->kset.rigidbody) return fail("%s is in kernel file 'rigidbody': add it to the kernel file list", who);
+    DevField *pos = particle_field(h, "position", 3, who);
+    DevField *vel = pos ? particle_field(h, "velocity", 3, who) : nullptr;
+    DevField *bn = vel ? particle_field(h, "body_number", 1, who) : nullptr;
+    DevField *bp = bn ? body_field(h, "body_position", body, who) : nullptr;
+    unsigned body_count = 0;
+    if (!bp || body_count_of(h, who, &body_count)) return -1;
+    sphk::StepParams p;
+    if (build_params(h, &p)) return -1;
+    if (ensure_current(h, pos) || ensure_current(h, vel) || ensure_current(h, bn)) return -1;
+    h->pairs.valid = 0;
+    if (timed(h, who, [&] {
+            return sphk::launch_set_body_position(static_cast<float *>(pos->ptr()), static_cast<float *>(vel->ptr()),
+                                                  static_cast<const unsigned *>(bn->ptr()), h->n, body,
+                                                  body < body_count, static_cast<float *>(bp->ptr()), position, p.dt,
+                                                  h->stream);
+        }))
+        return -1;
+    std::memcpy(h->shadow.data() + h->host.data_offsets[bp->index] + 12 * static_cast<size_t>(body), position, 12);
+    return 0;
+}
+
+extern "C" int set_body_rotation_device_hip(psdata_hip *h, unsigned body, const float *euler_angles) {
+    static const char *who = "set_body_rotation";
+    if (!h || !euler_angles) return fail("null argument");
+    if (!h->kset.rigidbody) return fail("%s is in kernel file 'rigidbody': add it to the kernel file list", who);
+    DevField *pos = particle_field(h, "position", 3, who);
+    DevField *rot = pos ? particle_field(h, "rotation", 9, who) : nullptr;
+    DevField *bn = rot ? particle_field(h, "body_number", 1, who) : nullptr;
+    DevField *be = bn ? body_field(h, "body_euler_rotation", body, who) : nullptr;
+    unsigned body_count = 0;
+    if (!be || body_count_of(h, who, &body_count)) return -1;
+    if (ensure_current(h, pos) || ensure_current(h, rot) || ensure_current(h, bn)) return -1;
+    // The body matrix is the one of the stored angles, read back as the device holds them.
+    float stored[3];
+    HIP_TRY(hipMemcpyAsync(stored, static_cast<char *>(be->ptr()) + 12 * static_cast<size_t>(body), 12,
+                           hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const sphk::Mat3 bt = transposed(euler_matrix(stored));
+    h->pairs.valid = 0;
+    if (timed(h, who, [&] {
+            return sphk::launch_set_body_rotation(static_cast<float *>(pos->ptr()), static_cast<float *>(rot->ptr()),
+                                                  static_cast<const unsigned *>(bn->ptr()), h->n, body,
+                                                  body < body_count, bt, static_cast<float *>(be->ptr()),
+                                                  euler_angles, h->stream);
+        }))
+        return -1;
+    std::memcpy(h->shadow.data() + h->host.data_offsets[be->index] + 12 * static_cast<size_t>(body), euler_angles,
+                12);
+    return 0;
 }
 
 extern "C" int global_sums_hip(psdata_hip *h, double *out) {

@@ -125,6 +125,11 @@ _reset_stats = _fn("reset_kernel_stats_hip", [c_void_p], c_int)
 _get_stats = _fn("get_kernel_stats_hip", [c_void_p, c_int, POINTER(c_char_p), POINTER(c_ulonglong),
                                           POINTER(c_double)], c_int)
 _global_sums = _fn("global_sums_hip", [c_void_p, POINTER(c_double)], c_int)
+_populate_cuboid = _fn("populate_position_cuboid_device_hip",
+                       [c_void_p, c_float, c_float, c_float, c_float, c_float, c_float, c_uint, c_uint, c_uint], c_int)
+_rotate_particles = _fn("rotate_particles_device_hip", [c_void_p, c_float, c_float, c_float], c_int)
+_set_body_position = _fn("set_body_position_device_hip", [c_void_p, c_uint, POINTER(c_float)], c_int)
+_set_body_rotation = _fn("set_body_rotation_device_hip", [c_void_p, c_uint, POINTER(c_float)], c_int)
 
 
 def _names_array(names):
@@ -247,6 +252,25 @@ class PsdataHip:
         out = (c_double * 7)()
         check(_global_sums(self.handle, out))
         return list(out[0:3]), out[3], list(out[4:7])
+
+    # initial-condition helpers and scripted rigid bodies
+    # (populate_position_cuboid_device_opencl, rotate_particles_device_opencl,
+    # set_body_position/rotation_device_opencl; scripts/sph/bindings.py)
+    def populate_position_cuboid(self, corner1, corner2, size):
+        c1 = [float(v) for v in corner1]
+        c2 = [float(v) for v in corner2]
+        check(_populate_cuboid(self.handle, *c1, *c2, *(int(v) for v in size)))
+
+    def rotate_particles(self, angle_x, angle_y, angle_z):
+        check(_rotate_particles(self.handle, float(angle_x), float(angle_y), float(angle_z)))
+
+    def set_body_position(self, body, position):
+        v = (c_float * 3)(*[float(x) for x in position])
+        check(_set_body_position(self.handle, int(body), v))
+
+    def set_body_rotation(self, body, euler_angles):
+        v = (c_float * 3)(*[float(x) for x in euler_angles])
+        check(_set_body_rotation(self.handle, int(body), v))
 
 
 # ---- scene batch ----------------------------------------------------------

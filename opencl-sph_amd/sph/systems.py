@@ -102,10 +102,10 @@ class ParticleSystem(object):
         self.psdata_opencl.sync_fields_device_to_host(self.psdata, list(args))
 
     def set_body_position(self, body, position):
-        raise bnd.SphError("scripted rigid bodies are not provided by this engine")
+        self.psdata_opencl.set_body_position(body, np.asarray(position).astype(np.dtype('f4')))
 
     def set_body_rotation(self, body, euler_angles):
-        raise bnd.SphError("scripted rigid bodies are not provided by this engine")
+        self.psdata_opencl.set_body_rotation(body, np.asarray(euler_angles).astype(np.dtype('f4')))
 
     def call_for_all_particles(self, kernel):
         self.psdata_opencl.call_for_all_particles(kernel)

@@ -238,3 +238,77 @@ def apq_times_inverse(apq, aqq_sym, precision="f64", fp64=True):
     out = np.zeros(9, dt)
     lib.orc_jacobi_inverse_apply(_ptr(a), _ptr(q), _ptr(out), int(fp64))
     return out.reshape(3, 3)
+
+
+# -- scripted rigid bodies and initial-condition helpers ----------------------
+# (kernels/common.cl:533-604, kernels/rigidbody.cl:1-120).  Arrays are in
+# particle-index order; every function returns new arrays.
+def _arr(a, dt, shape=None):
+    out = np.array(a, dtype=dt, copy=True)
+    return np.ascontiguousarray(out if shape is None else out.reshape(shape))
+
+
+def _real_args(lib, real, name, spec):
+    fn = getattr(lib, name)
+    fn.argtypes = [real if t == "r" else (c_uint if t == "u" else c_void_p) for t in spec]
+    return fn
+
+
+def populate_position_cuboid(position, pnum, n, corner1, corner2, size, precision="f32"):
+    """(position, n) after populate_position_cuboid; position has pnum rows."""
+    lib, real, _ = load(precision)
+    dt = np.float32 if precision == "f32" else np.float64
+    pos = _arr(position, dt, (int(pnum), 3))
+    nn = np.array([n], np.uint32)
+    c1, c2 = _arr(corner1, dt, 3), _arr(corner2, dt, 3)
+    fn = _real_args(lib, real, "orc_populate_cuboid", "pupppuuu")
+    fn(_ptr(pos), int(pnum), _ptr(nn), _ptr(c1), _ptr(c2), *(int(v) for v in size))
+    return pos, int(nn[0])
+
+
+def rotate_particles(position, n, angle_x, angle_y, angle_z, precision="f32"):
+    lib, real, _ = load(precision)
+    dt = np.float32 if precision == "f32" else np.float64
+    pos = _arr(position, dt)
+    pos = pos.reshape(-1, 3)
+    fn = _real_args(lib, real, "orc_rotate_particles", "purrr")
+    fn(_ptr(pos), int(n), float(angle_x), float(angle_y), float(angle_z))
+    return pos
+
+
+def euler_matrix(angles, precision="f64"):
+    """Body matrix of Euler angles (x, y, x2) as get_matrix_from_euler forms it."""
+    lib, real, _ = load(precision)
+    dt = np.float32 if precision == "f32" else np.float64
+    e = _arr(angles, dt, 3)
+    out = np.zeros(9, dt)
+    lib.orc_euler_matrix.argtypes = [c_void_p, c_void_p]
+    lib.orc_euler_matrix(_ptr(e), _ptr(out))
+    return out.reshape(3, 3)
+
+
+def set_body_position(position, velocity, body_number, n, body_count, dt_step, body, body_position, target,
+                      precision="f32"):
+    """(position, velocity, body_position) after set_body_position."""
+    lib, real, _ = load(precision)
+    dt = np.float32 if precision == "f32" else np.float64
+    pos, vel = _arr(position, dt, (-1, 3)), _arr(velocity, dt, (-1, 3))
+    bn = _arr(body_number, np.uint32)
+    bp = _arr(body_position, dt, (-1, 3))
+    t = _arr(target, dt, 3)
+    fn = _real_args(lib, real, "orc_set_body_position", "pppuurupp")
+    fn(_ptr(pos), _ptr(vel), _ptr(bn), int(n), int(body_count), float(dt_step), int(body), _ptr(bp), _ptr(t))
+    return pos, vel, bp
+
+
+def set_body_rotation(position, rotation, body_number, n, body_count, body, body_euler, angles, precision="f32"):
+    """(position, rotation, body_euler_rotation) after set_body_rotation."""
+    lib, real, _ = load(precision)
+    dt = np.float32 if precision == "f32" else np.float64
+    pos, rot = _arr(position, dt, (-1, 3)), _arr(rotation, dt, (-1, 9))
+    bn = _arr(body_number, np.uint32)
+    be = _arr(body_euler, dt, (-1, 3))
+    a = _arr(angles, dt, 3)
+    fn = _real_args(lib, real, "orc_set_body_rotation", "pppuuupp")
+    fn(_ptr(pos), _ptr(rot), _ptr(bn), int(n), int(body_count), int(body), _ptr(be), _ptr(a))
+    return pos, rot, be

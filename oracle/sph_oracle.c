@@ -775,6 +775,131 @@ void orc_forces_solids(const orc_params *p, const real *pos, const real *vel, co
     }
 }
 
+/* ---- Scripted rigid bodies and initial-condition helpers ------------------
+ * populate_position_cuboid and rotate_particles: kernels/common.cl:533-604;
+ * set_body_position and set_body_rotation: kernels/rigidbody.cl:1-120.
+ * Matrices are row-major 3x3.  Products follow multiplyMatrices
+ * (kernels/common.cl:208-220): o[r][c] = (a[r][0] b[0][c] + a[r][1] b[1][c])
+ * + a[r][2] b[2][c], zero terms included. */
+static real rcos(real x) { return sizeof(real) == 4 ? (real)cosf((float)x) : (real)cos((double)x); }
+static real rsin(real x) { return sizeof(real) == 4 ? (real)sinf((float)x) : (real)sin((double)x); }
+
+static void mm3(const real *a, const real *b, real *o) {
+    int r, c;
+    for (r = 0; r < 3; ++r)
+        for (c = 0; c < 3; ++c) o[3 * r + c] = a[3 * r] * b[c] + a[3 * r + 1] * b[3 + c] + a[3 * r + 2] * b[6 + c];
+}
+static void axis_x(real a, real *m) {
+    real c = rcos(a), s = rsin(a);
+    m[0] = 1; m[1] = 0; m[2] = 0;
+    m[3] = 0; m[4] = c; m[5] = -s;
+    m[6] = 0; m[7] = s; m[8] = c;
+}
+/* The reference's y factor: sin enters as [0][2] = -s, [2][0] = +s. */
+static void axis_y(real a, real *m) {
+    real c = rcos(a), s = rsin(a);
+    m[0] = c; m[1] = 0; m[2] = -s;
+    m[3] = 0; m[4] = 1; m[5] = 0;
+    m[6] = s; m[7] = 0; m[8] = c;
+}
+static void axis_z(real a, real *m) {
+    real c = rcos(a), s = rsin(a);
+    m[0] = c; m[1] = -s; m[2] = 0;
+    m[3] = s; m[4] = c; m[5] = 0;
+    m[6] = 0; m[7] = 0; m[8] = 1;
+}
+
+/* rotate_particles: x <- Rx (Ry Rz) x for the first n particles
+ * (kernels/common.cl:561-604). */
+void orc_rotate_particles(real *pos, unsigned n, real ax, real ay, real az) {
+    real mx[9], my[9], mz[9], myz[9], m[9];
+    unsigned i;
+    axis_x(ax, mx);
+    axis_y(ay, my);
+    axis_z(az, mz);
+    mm3(my, mz, myz);
+    mm3(mx, myz, m);
+    for (i = 0; i < n; ++i) st(pos, i, mat_mv(m, ld(pos, i)));
+}
+
+/* Body matrix of Euler angles (x, y, x2): Rx(x2) (Ry(y) Rx(x))
+ * (get_matrix_from_euler, kernels/rigidbody.cl:35-66). */
+void orc_euler_matrix(const real *e, real *out) {
+    real m1[9], m2[9], m3[9], t[9];
+    axis_x(e[0], m1);
+    axis_y(e[1], m2);
+    axis_x(e[2], m3);
+    mm3(m2, m1, t);
+    mm3(m3, t, out);
+}
+
+/* populate_position_cuboid: one lane per lattice point (i, j, k), particle
+ * index i + sx j + sx sy k in 32-bit unsigned arithmetic; lanes at or past
+ * pnum write nothing; the lane of particle pnum - 1 sets n = pnum, else the
+ * last lattice point sets n = sx sy sz (kernels/common.cl:535-559). */
+void orc_populate_cuboid(real *pos, unsigned pnum, unsigned *n, const real *c1, const real *c2, unsigned sx,
+                         unsigned sy, unsigned sz) {
+    unsigned i, j, k;
+    for (k = 0; k < sz; ++k)
+        for (j = 0; j < sy; ++j)
+            for (i = 0; i < sx; ++i) {
+                const unsigned p = i + sx * j + sx * sy * k;
+                vec v;
+                if (p >= pnum) continue;
+                if (p == pnum - 1) *n = pnum;
+                else if (i == sx - 1 && j == sy - 1 && k == sz - 1) *n = sx * sy * sz;
+                v.x = ((real)i / (real)(sx - 1)) * (c2[0] - c1[0]) + c1[0];
+                v.y = ((real)j / (real)(sy - 1)) * (c2[1] - c1[1]) + c1[1];
+                v.z = ((real)k / (real)(sz - 1)) * (c2[2] - c1[2]) + c1[2];
+                st(pos, p, v);
+            }
+}
+
+/* set_body_position: particles of body b (body_number == b + 1, b below
+ * body_count) move by new - old and get velocity (new - old) / dt; the body's
+ * stored position becomes new (kernels/rigidbody.cl:1-31).  Every particle
+ * reads the old body position. */
+void orc_set_body_position(real *pos, real *vel, const unsigned *body_number, unsigned n, unsigned body_count,
+                           real dt, unsigned body, real *body_position, const real *newp) {
+    const vec old = ld(body_position, body);
+    const vec nw = {newp[0], newp[1], newp[2]};
+    unsigned i;
+    if (body < body_count)
+        for (i = 0; i < n; ++i) {
+            if (body_number[i] != body + 1) continue;
+            st(pos, i, vadd(vsub(ld(pos, i), old), nw));
+            st(vel, i, vdivs(vsub(nw, old), dt));
+        }
+    st(body_position, body, nw);
+}
+
+/* set_body_rotation: with B the matrix of the body's stored Euler angles and
+ * P a particle's rotation field, T = B^T P; the position becomes T x and the
+ * rotation field T P; the stored angles become the new ones
+ * (kernels/rigidbody.cl:68-120). */
+void orc_set_body_rotation(real *pos, real *rot, const unsigned *body_number, unsigned n, unsigned body_count,
+                           unsigned body, real *body_euler, const real *euler_new) {
+    real b[9], bt[9];
+    unsigned i;
+    int r, c;
+    orc_euler_matrix(body_euler + 3 * (size_t)body, b);
+    for (r = 0; r < 3; ++r)
+        for (c = 0; c < 3; ++c) bt[3 * r + c] = b[3 * c + r];
+    if (body < body_count)
+        for (i = 0; i < n; ++i) {
+            real t[9], t2[9];
+            real *p = rot + 9 * (size_t)i;
+            if (body_number[i] != body + 1) continue;
+            mm3(bt, p, t);
+            st(pos, i, mat_mv(t, ld(pos, i)));
+            mm3(t, p, t2);
+            memcpy(p, t2, sizeof t2);
+        }
+    body_euler[3 * (size_t)body] = euler_new[0];
+    body_euler[3 * (size_t)body + 1] = euler_new[1];
+    body_euler[3 * (size_t)body + 2] = euler_new[2];
+}
+
 unsigned orc_real_size(void) { return (unsigned)sizeof(real); }
 
 #ifdef _OPENMP
