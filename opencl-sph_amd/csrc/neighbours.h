@@ -52,20 +52,29 @@ __device__ __forceinline__ void for_each_candidate(const GridGeom &g, unsigned l
 // together lets their memory latency overlap; the order in which use()
 // sees the entries, and so every sum, is the list order.  Entries past the
 // end read slot i's own data (load must accept it) and are not used.
+// The next group's entries are loaded while the current group is used, so
+// only the first group waits for its list entries.
 template <unsigned G, typename Load, typename Use>
 __device__ __forceinline__ void for_each_listed(const PairCache &c, unsigned i, unsigned cnt, Load &&load,
                                                 Use &&use) {
-    for (unsigned k0 = 0; k0 < cnt; k0 += G) {
-        unsigned e[G];
+    if (cnt == 0) return;
+    const unsigned *__restrict__ col = c.list + i;
+    const size_t stride = c.stride;
+    unsigned e[G];
 #pragma unroll
-        for (unsigned g = 0; g < G; ++g)
-            e[g] = k0 + g < cnt ? c.list[static_cast<size_t>(k0 + g) * c.stride + i] : i;
+    for (unsigned g = 0; g < G; ++g) e[g] = g < cnt ? col[g * stride] : i;
+    for (unsigned k0 = 0; k0 < cnt; k0 += G) {
         decltype(load(0u)) d[G];
 #pragma unroll
         for (unsigned g = 0; g < G; ++g) d[g] = load(e[g]);
+        unsigned nx[G];
+#pragma unroll
+        for (unsigned g = 0; g < G; ++g) nx[g] = k0 + G + g < cnt ? col[(k0 + G + g) * stride] : i;
 #pragma unroll
         for (unsigned g = 0; g < G; ++g)
             if (k0 + g < cnt) use(e[g], d[g]);
+#pragma unroll
+        for (unsigned g = 0; g < G; ++g) e[g] = nx[g];
     }
 }
 

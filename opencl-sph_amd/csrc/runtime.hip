@@ -88,6 +88,7 @@ struct psdata_hip {
     double *sums = nullptr;                 // partials + 7 outputs
     sphk::PairCache pairs{};                // neighbour lists + pressures (fluid programs)
     int compensated = 0, polar_fp64 = 1;
+    unsigned long long bins_epoch = 1;      // slot-order epoch the grid arrays index
     bool profiling = false;
     std::map<std::string, KernelStat> stats;
     std::vector<PendingTiming> pending;
@@ -278,6 +279,7 @@ int compute_bins(psdata_hip *h) {
         vel->epoch = h->epoch;
     }
     gcell->epoch = h->epoch;   // written in the new order in place
+    h->bins_epoch = h->epoch;
     return 0;
 }
 
@@ -319,6 +321,9 @@ int check_field_shapes(psdata_hip *h) {
 
 int check_and_prepare(psdata_hip *h, const KernelSpec &k) {
     if (k.neighbours && !h->has_grid) return fail("%s needs the grid arrays", k.name);
+    if (k.neighbours && h->bins_epoch != h->epoch)
+        return fail("%s: the particle order changed since the last compute_particle_bins (n changed or the "
+                    "positions were re-placed); call compute_particle_bins first", k.name);
     for (const FieldUse &u : k.uses) {
         DevField *f = h->field(u.name);
         if (!f) {
