@@ -283,7 +283,8 @@ size_t pool_bytes(size_t slots, size_t cells, size_t scenes) {
     const size_t per_cell = 2 * 4;
     const size_t tiles = (cells + sphk::kScanTile - 1) / sphk::kScanTile + 1;
     const size_t per_scene = sizeof(StepParams) + 4 + sphk::kBodiesPerScene * 4 + sizeof(Segment) * 2 + 4 * 8;
-    return slots * per_slot + cells * per_cell + tiles * 4 + scenes * per_scene;
+    const size_t scan_pad = 2 * 2 * 3 * 4 * sphk::kScanPad;               // padding of position, reference position
+    return slots * per_slot + cells * per_cell + tiles * 4 + scenes * per_scene + scan_pad;
 }
 
 void release(sph_batch *b) {
@@ -689,7 +690,8 @@ extern "C" int sph_batch_create(unsigned slots, unsigned cells, unsigned scenes,
     if (!out) return fail("sph_batch_create: out is NULL");
     *out = nullptr;
     if (slots == 0 || cells == 0 || scenes == 0) return fail("sph_batch_create: empty pool");
-    if (slots > sphk::kListSlot) return fail("sph_batch_create: at most %u slots", sphk::kListSlot);
+    if (slots > sphk::kListSlot || slots > sphk::kMaxSlots)
+        return fail("sph_batch_create: at most %u slots", std::min(sphk::kListSlot, sphk::kMaxSlots));
     sph_batch *b = new sph_batch();
     if (sphhip::current_device(&b->device)) {
         delete b;
@@ -709,7 +711,8 @@ extern "C" int sph_batch_create(unsigned slots, unsigned cells, unsigned scenes,
     int rc = 0;
     for (int k = 0; k < 2 && !rc; ++k) {
         SlotState &st = b->st[k];
-        rc = dev_alloc(b, &st.pos, 3 * S) || dev_alloc(b, &st.vel, 3 * S) || dev_alloc(b, &st.opos, 3 * S) ||
+        rc = dev_alloc(b, &st.pos, 3 * (S + sphk::kScanPad)) || dev_alloc(b, &st.vel, 3 * S) ||
+             dev_alloc(b, &st.opos, 3 * (S + sphk::kScanPad)) ||
              dev_alloc(b, &st.d0, S) || dev_alloc(b, &st.rot, 9 * S) || dev_alloc(b, &st.strain, 6 * S) ||
              dev_alloc(b, &st.body, S) || dev_alloc(b, &st.id, S);
     }

@@ -26,9 +26,26 @@ struct __attribute__((packed, aligned(4))) Packed3 {
 };
 
 __device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+// Per-slot fields stay below 4 GiB (checked when a handle or a batch is
+// created), so a slot's byte offset is a 32-bit value and a load is
+// "scalar base + 32-bit vector offset", with no 64-bit address arithmetic.
+template <typename T>
+__device__ __forceinline__ const T *at_offset(const T *p, unsigned byte_offset) {
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + byte_offset);
+}
 __device__ __forceinline__ V3 load3(const float *p, unsigned i) {
-    const Packed3 t = *reinterpret_cast<const Packed3 *>(p + 3 * static_cast<size_t>(i));
+    const Packed3 t = *reinterpret_cast<const Packed3 *>(at_offset(p, i * 12u));
     return V3{t.x, t.y, t.z};
+}
+// The 12-byte tuple at a byte offset (the scans step these by 12 per slot).
+__device__ __forceinline__ V3 load3_at(const float *p, unsigned byte_offset) {
+    const Packed3 t = *reinterpret_cast<const Packed3 *>(at_offset(p, byte_offset));
+    return V3{t.x, t.y, t.z};
+}
+// Element i of a per-slot array of 32-bit values.
+template <typename T>
+__device__ __forceinline__ T load1(const T *p, unsigned i) {
+    return *at_offset(p, i * 4u);
 }
 __device__ __forceinline__ void store3(float *p, unsigned i, V3 v) {
     p[3 * i] = v.x;

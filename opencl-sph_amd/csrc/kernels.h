@@ -26,6 +26,12 @@ struct GridGeom {
 
 enum SystemKind : unsigned { kFluid = 0, kCorotational = 1, kSvk = 2 };
 constexpr unsigned kSceneSetup = 1u;
+// Position and reference-position buffers hold kScanPad slots past the last
+// one: the unrolled candidate scans load up to kScanPad - 1 slots beyond a
+// row's end (never used) instead of clamping every index.
+constexpr unsigned kScanPad = 8;
+// Slot byte offsets of 12-byte tuples are 32-bit values.
+constexpr unsigned kMaxSlots = 0xffffffffu / 12u - kScanPad;
 
 // Everything the per-particle passes read besides the per-particle fields.
 // A single psdata is one scene at slots [0, n); a batch holds one StepParams

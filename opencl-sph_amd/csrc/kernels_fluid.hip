@@ -170,12 +170,15 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
             unsigned b, e;
             if (!row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e)) continue;
             if (e - b > (1u << kRowBits)) fast = false;
-            // kScanUnroll candidates per round, loads first (the last valid
-            // slot stands in past the end), tested in slot order
+            // kScanUnroll candidates per round, loads first (slots past the
+            // row's end are read, within the buffer's kScanPad padding, and
+            // not used), tested in slot order
+            static_assert(kScanUnroll <= kScanPad, "scan rounds read past the row end");
             for (unsigned j0 = b; j0 < e; j0 += kScanUnroll) {
                 V3 xs[kScanUnroll];
+                const unsigned off = j0 * 12u;
 #pragma unroll
-                for (unsigned u = 0; u < kScanUnroll; ++u) xs[u] = load3(pos, min(j0 + u, e - 1u));
+                for (unsigned u = 0; u < kScanUnroll; ++u) xs[u] = load3_at(pos, off + 12u * u);
 #pragma unroll
                 for (unsigned u = 0; u < kScanUnroll; ++u) {
                     if (j0 + u >= e) break;
@@ -280,13 +283,14 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                 unsigned b, e;
                 if (!row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e)) continue;
                 if (e - b > (1u << kSolidOffsetBits)) fast = false;
+                static_assert(kSolidScanUnroll <= kScanPad, "scan rounds read past the row end");
                 for (unsigned j0 = b; j0 < e; j0 += kSolidScanUnroll) {
                     V3 xs[kSolidScanUnroll], x0s[kSolidScanUnroll];
+                    const unsigned off = j0 * 12u;
 #pragma unroll
                     for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
-                        const unsigned j = min(j0 + u, e - 1u);
-                        xs[u] = load3(pos, j);
-                        x0s[u] = load3(opos, j);
+                        xs[u] = load3_at(pos, off + 12u * u);
+                        x0s[u] = load3_at(opos, off + 12u * u);
                     }
 #pragma unroll
                     for (unsigned u = 0; u < kSolidScanUnroll; ++u) {

@@ -875,6 +875,10 @@ extern "C" int create_psdata_hip(psdata *data, const char *file_list, psdata_hip
         release(h);
         return fail("n = %u exceeds pnum = %u", h->n, h->pnum);
     }
+    if (h->pnum > sphk::kMaxSlots) {
+        release(h);
+        return fail("pnum = %u exceeds the engine's limit of %u particles", h->pnum, sphk::kMaxSlots);
+    }
     std::vector<FieldKind> kinds = sphhip::field_kinds_for(data->data);
     const unsigned total = psdata_data_size(*data);
     h->shadow.assign(static_cast<const char *>(data->data), static_cast<const char *>(data->data) + total);
@@ -902,7 +906,10 @@ extern "C" int create_psdata_hip(psdata *data, const char *file_list, psdata_hip
                 max_particle_bytes = std::max(max_particle_bytes, f.bytes);
             }
         }
-        const size_t alloc = std::max<size_t>(f.bytes, 4);
+        // per-slot fields carry kScanPad slots of padding (read, never used,
+        // by the unrolled candidate scans)
+        const size_t alloc = std::max<size_t>(
+            f.bytes + (f.kind == FieldKind::Particle ? static_cast<size_t>(sphk::kScanPad) * f.words * 4 : 0), 4);
         if (hipMalloc(&f.buf[0], alloc) != hipSuccess ||
             (f.hot && hipMalloc(&f.buf[1], alloc) != hipSuccess)) {
             release(h);
