@@ -30,8 +30,9 @@ constexpr unsigned kSceneSetup = 1u;
 // one: the unrolled candidate scans load up to kScanPad - 1 slots beyond a
 // row's end (never used) instead of clamping every index.
 constexpr unsigned kScanPad = 8;
-// Slot byte offsets of 12-byte tuples are 32-bit values.
-constexpr unsigned kMaxSlots = 0xffffffffu / 12u - kScanPad;
+// Slot byte offsets of the widest per-slot record loaded with a 32-bit
+// offset (a 3x3 rotation, 36 bytes) are 32-bit values.
+constexpr unsigned kMaxSlots = 0xffffffffu / 36u - kScanPad;
 
 // Everything the per-particle passes read besides the per-particle fields.
 // A single psdata is one scene at slots [0, n); a batch holds one StepParams

@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
             float rho, p;
         };
         for_each_listed<kPairGroup>(
-            c, i, cnt, [&](unsigned j) { return Nbr{load3(pos, j), load3(vel, j), rho[j], prs[j]}; },
+            c, i, cnt, [&](unsigned j) { return Nbr{load3(pos, j), load3(vel, j), load1(rho, j), load1(prs, j)}; },
             [&](unsigned j, const Nbr &n) {
                 if (j != i) acc.pair(visc, kc, xi, vi, rhoi, pi, n.x, n.v, n.rho, n.p);
             });

@@ -23,6 +23,7 @@ constexpr unsigned kBlock = 256;
 // deformation sweeps load two positions per entry, the force pass ~27 floats.
 constexpr unsigned kSweepGroup = 8;
 constexpr unsigned kForceGroup = 4;
+constexpr unsigned kForceGroupCoro = 3;   // + 9 rotation floats per entry: 3 keeps 3 waves per SIMD
 
 template <bool KAHAN>
 struct Acc {
@@ -401,7 +402,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, Pa
         V3 g, gm;   // grad_spiky(X), grad_spiky(-X)
         grad_spiky_pm(X, kc, &g, &gm);
         const V3 fji = -vivj * sym_mul(si, g);
-        const V3 fij = (f.body_number && f.body_number[j] > 0u) ? -fji : -vivj * sym_mul(sj, gm);
+        const V3 fij = (f.body_number && load1(f.body_number, j) > 0u) ? -fji : -vivj * sym_mul(sj, gm);
         const V3 e = CORO ? (-mat_mul(ri, fji)) + mat_mul(rj, fij) : (-fji) + fij;
         const V3 v = (mrho * (vj - vi)) * lap_visc(rcur, kc);
         fe.add(e);
@@ -414,7 +415,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, Pa
             V3 x, x0, v;
             float d0, rho, s[6], r[CORO ? 9 : 1];
         };
-        for_each_listed<kForceGroup>(
+        for_each_listed<(CORO ? kForceGroupCoro : kForceGroup)>(
             c, i, cnt,
             [&](unsigned e) {
                 const unsigned j = e & kListSlot;
@@ -422,11 +423,14 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, Pa
                 n.x = load3(pos, j);
                 n.x0 = load3(opos, j);
                 n.v = load3(vel, j);
-                n.d0 = f.density0[j];
-                n.rho = f.density[j];
-                for (int m = 0; m < 6; ++m) n.s[m] = stress[6 * j + m];
-                if (CORO)
-                    for (int m = 0; m < 9; ++m) n.r[m] = rot[9 * j + m];
+                n.d0 = load1(f.density0, j);
+                n.rho = load1(f.density, j);
+                const float *sj = at_offset(stress, j * 24u);
+                for (int m = 0; m < 6; ++m) n.s[m] = sj[m];
+                if (CORO) {
+                    const float *rj = at_offset(rot, j * 36u);
+                    for (int m = 0; m < 9; ++m) n.r[m] = rj[m];
+                }
                 return n;
             },
             [&](unsigned e, const Nbr &n) {
