@@ -210,9 +210,21 @@ def run_single(args):
             roof["valu_tflops"] = round(flops / avg_s / 1e12, 3)
             roof["valu_frac"] = round(flops / avg_s / 1e12 / roofline.MI355X_FP32_TFLOPS, 4)
         t_roof = roofline.t_roof_seconds(scene.n, cells, candidates, in_range)
+        # the streaming (HBM-bound) kernels against the HBM peak, each on its
+        # own algorithmic bytes
+        streaming = {}
+        for k in ("bin_key_count", "bin_scan", "bin_scatter", "bin_rank_place", "step_forward"):
+            if k not in stats:
+                continue
+            n_launch, ms = stats[k]
+            b = roofline.bytes_per_launch(k, scene.n, cells)
+            gbs = b / (ms / n_launch / 1e3) / 1e9
+            streaming[k] = {"bytes_per_launch": b, "avg_launch_ms": round(ms / n_launch, 5),
+                            "achieved_gbs": round(gbs, 1), "frac": round(gbs / roofline.MI355X_HBM_GBS, 4)}
         extra = {"step_roofline": {"t_roof_ms": round(t_roof * 1e3, 5), "frac": round(t_roof * 1e3 / ms_per_step, 4),
                                    "algorithmic_bytes_per_step": roofline.step_bytes(scene.n, cells),
                                    "candidate_pairs": candidates, "in_range_pairs": in_range},
+                 "hbm_kernels": streaming,
                  "kernel_ms_per_step": {k: round(v, 5) for k, v in per_step.items()}}
     system.free()
 

@@ -169,16 +169,49 @@ __device__ __forceinline__ T det3(const T *m) {
            m[2] * (m[3] * m[7] - m[4] * m[6]);
 }
 
-__device__ __forceinline__ float t_atan(float x) { return atanf(x); }
-__device__ __forceinline__ double t_atan(double x) { return atan(x); }
-__device__ __forceinline__ float t_sin(float x) { return sinf(x); }
-__device__ __forceinline__ double t_sin(double x) { return sin(x); }
-__device__ __forceinline__ float t_cos(float x) { return cosf(x); }
-__device__ __forceinline__ double t_cos(double x) { return cos(x); }
 __device__ __forceinline__ float t_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double t_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float t_abs(float x) { return fabsf(x); }
 __device__ __forceinline__ double t_abs(double x) { return fabs(x); }
+
+// cos and sin of angle = 1/2 atan(e / d).  FP32: the reference's library
+// calls on the reference's quotient.  FP64 (the precision upgrade, not bound
+// to the reference's bits): the same angle without library trig, from
+// r = sqrt(d^2 + e^2): cos phi = |d| / r, sin phi = sign(d) e / r (phi in
+// (-pi/2, pi/2), the range of atan; sign(-0) = -1 as e / -0 = -inf has it),
+// cos(phi/2) = sqrt((1 + cos phi) / 2) >= cos(pi/4) and
+// sin(phi/2) = sin phi / (2 cos(phi/2)).  d and e come from FP32 data, so
+// d^2 + e^2 cannot overflow; a non-finite r (infinite or NaN input) takes
+// the quotient form.  Agrees with the library calls to a few FP64 ulps.
+__device__ __forceinline__ void half_atan_cs(float e, float d, float *c, float *s) {
+    const float angle = 0.5f * atanf(e / d);
+    *c = cosf(angle);
+    *s = sinf(angle);
+}
+__device__ __forceinline__ void half_atan_cs(double e, double d, double *c, double *s) {
+    const double r = sqrt(d * d + e * e);
+    double cphi, sphi;
+    if (r <= 1.7e308) {
+        const double inv = 1.0 / r;
+        cphi = fabs(d) * inv;
+        sphi = (signbit(d) ? -e : e) * inv;
+    } else {
+        const double x = e / d;
+        if (fabs(x) <= 1.0) {
+            const double q = 1.0 / sqrt(1.0 + x * x);
+            cphi = q;
+            sphi = x * q;
+        } else {
+            const double ix = 1.0 / fabs(x);
+            const double q = 1.0 / sqrt(1.0 + ix * ix);
+            cphi = ix * q;
+            sphi = copysign(q, x);
+        }
+    }
+    const double ch = sqrt(0.5 + 0.5 * cphi);
+    *c = ch;
+    *s = sphi / (ch + ch);
+}
 
 // One Givens rotation that zeroes off-diagonal `od` (3, 4 or 5) of the
 // symmetric `a`, applied to the eigenvector columns of `p`
@@ -193,9 +226,8 @@ __device__ __forceinline__ void givens(const T *a, const T *p, int od, T *ao, T 
     const int iee = 2 - k;
     const int iex = k == 0 ? 4 : 3;
     const int iey = k == 2 ? 4 : 5;
-    T angle = T(0);
-    if (a[od] != T(0)) angle = T(0.5) * t_atan(T(2) * a[od] / (a[ixx] - a[iyy]));
-    const T c = t_cos(angle), s = t_sin(angle);
+    T c = T(1), s = T(0);   // cos(0), sin(0)
+    if (a[od] != T(0)) half_atan_cs(T(2) * a[od], a[ixx] - a[iyy], &c, &s);
     const T c2 = c * c, s2 = s * s, cs = c * s;
     ao[ixx] = a[ixx] * c2 + a[iyy] * s2 + T(2) * a[od] * cs;
     ao[od] = -a[ixx] * cs + a[iyy] * cs + a[od] * (c2 - s2);
