@@ -407,8 +407,10 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, Pa
         const V3 v = (mrho * (vj - vi)) * lap_visc(rcur, kc);
         fe.add(e);
         fv.add(v);
-        fec.add(e);
-        fvc.add(v);
+        if (MODE != kListedOnly) {   // listed lanes have no mirror images: cutoff sums = full sums
+            fec.add(e);
+            fvc.add(v);
+        }
     };
     if (MODE == kListedOnly) {
         struct Nbr {
@@ -491,7 +493,11 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, Pa
     store3(f.force, i, e + v);
     if (f.force_elastic) store3(f.force_elastic, i, e);
     if (f.force_viscous) store3(f.force_viscous, i, v);
-    if (f.cutoff_force) store3(f.cutoff_force, i, fec.value() * 0.5f + fvc.value() * p.viscosity);
+    if (f.cutoff_force) {
+        const V3 ec = MODE == kListedOnly ? fe.value() : fec.value();
+        const V3 vc = MODE == kListedOnly ? fv.value() : fvc.value();
+        store3(f.cutoff_force, i, ec * 0.5f + vc * p.viscosity);
+    }
 }
 
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
