@@ -61,6 +61,7 @@ struct StepParams {
     unsigned body_base;                            // first entry of body_dynamic for this scene
     int eos_tait;                                  // conf field eos_tait: Tait EOS (exponent 7)
     int mass_consistent;                           // conf field mass_consistent: fluid force times m
+    int svk_szz_zero;                              // conf field svk_sigma_zz_zero: the reference's sigma_zz
     int compensated;                               // kKahanDensity | kKahanForces
     int polar_fp64;                                // FP64 polar decomposition / inverse
 };

@@ -364,7 +364,9 @@ __global__ void __launch_bounds__(kBlock) k_stresses_svk(PS ps, FieldPtrs f) {
     float *s = f.stress + 6 * i;
     s[S00] = sc * cauchy[0];
     s[S11] = sc * cauchy[4];
-    s[S22] = sc * cauchy[8];
+    // svk_sigma_zz_zero: the value the reference stores (its sigma_zz is read
+    // past the end of a float[6], kernels/solid_svk.cl:48,75, and comes out 0)
+    s[S22] = p.svk_szz_zero ? 0.0f : sc * cauchy[8];
     s[S01] = sc * cauchy[1];
     s[S02] = sc * cauchy[2];
     s[S12] = sc * cauchy[5];

@@ -811,6 +811,7 @@ int params_from_blob(const psdata &meta, const char *blob, const KernelSet &ks, 
     p.eos_tait = r.read_u32(r.index("eos_tait"), 0, &flag) && flag != 0u;
     flag = 0;
     p.mass_consistent = r.read_u32(r.index("mass_consistent"), 0, &flag) && flag != 0u;
+    p.svk_szz_zero = r.read_u32(r.index("svk_sigma_zz_zero"), 0, &flag) && flag != 0u;
     p.system = ks.corot ? sphk::kCorotational : ks.svk ? sphk::kSvk : sphk::kFluid;
     p.compensated = compensated;
     p.polar_fp64 = polar_fp64;

@@ -32,7 +32,7 @@ def _params_type(real):
             ("nplanes", c_uint), ("planes", (real * 6) * MAXP),
             ("nconstraints", c_uint), ("constraints", (real * 6) * MAXC),
             ("body_count", c_uint), ("polar_fp64", c_int),
-            ("eos_tait", c_int), ("mass_consistent", c_int),
+            ("eos_tait", c_int), ("mass_consistent", c_int), ("svk_szz_zero", c_int),
         ]
     return Params
 
@@ -97,6 +97,7 @@ class OracleState:
         p.polar_fp64 = int(params.get("polar_fp64", 1))
         p.eos_tait = int(params.get("eos_tait", 0))
         p.mass_consistent = int(params.get("mass_consistent", 0))
+        p.svk_szz_zero = int(params.get("svk_sigma_zz_zero", 0))
         self.params = p
         self.ncells = res[0] * res[1] * res[2]
         n = self.n

@@ -52,6 +52,8 @@ typedef struct {
     int polar_fp64;
     int eos_tait;         /* 1: p = k ((rho/rho0)^7 - 1) instead of k (rho - rho0) */
     int mass_consistent;  /* 1: fluid force times the particle's own mass */
+    int svk_szz_zero;     /* 1: SVK sigma_zz stored as 0, what the reference's out-of-bounds read of a
+                             float[6] gives on its platform (kernels/solid_svk.cl:48,75) */
 } orc_params;
 
 typedef struct {
@@ -693,7 +695,7 @@ void orc_stresses_svk(const orc_params *p, const real *def, real *stress) {
         real *s = stress + 6 * i;
         s[0] = sc * cy[0];
         s[1] = sc * cy[4];
-        s[2] = sc * cy[8];
+        s[2] = p->svk_szz_zero ? (real)0 : sc * cy[8];
         s[3] = sc * cy[1];
         s[4] = sc * cy[2];
         s[5] = sc * cy[5];

@@ -77,8 +77,12 @@ class OracleSolid:
                   "gravity": g("gravity").reshape(-1).tolist(), "polar_fp64": 1,
                   "symmetry_planes": g("symmetry_planes").reshape(-1, 6),
                   "plane_constraints": g("plane_constraints").reshape(-1, 6)}
+        if bnd.get_field_psdata(self.psdata, b"svk_sigma_zz_zero") >= 0:
+            params["svk_sigma_zz_zero"] = int(g("svk_sigma_zz_zero")[0])
         names = ["position", "velocity", "originalpos", "density", "density0", "force", "stress"]
         names += ["rotation", "strain"] if self.system == "corotational" else ["deformation"]
+        names += [k for k in ("force_elastic", "force_viscous", "cutoff_force")
+                  if bnd.get_field_psdata(self.psdata, k.encode()) >= 0]
         fields = {k: g(k)[:n].reshape(n, -1) for k in names}
         fields["body_number"] = g("body_number")[:n]
         fields["body_dynamic"] = g("body_dynamic")
