@@ -108,34 +108,42 @@ class _Preparer:
 
 def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entries=4096, graph_steps=8,
                   compensated_sums=1, polar_fp64=True, realtime=True, on_result=None, fields=("position",),
-                  prepare_threads=2, prepare_ahead=64, profile=False, log=None):
-    """Continuous batching over one device pool.
+                  prepare_threads=2, prepare_ahead=64, profile=False, pools=2, log=None):
+    """Continuous batching over device pools.
 
-    The pool is sized to the HBM budget (or to `slots` / `cells` when given,
-    within the budget).  Worker threads generate the scenes of arrived
-    requests and build their psdata while the device steps.  Each round of
-    the loop admits every prepared request that fits (in arrival order,
-    looking past ones that do not fit yet), runs `graph_steps` batch steps,
-    then reads back and releases the scenes that finished.
+    Each pool is sized to its share of the HBM budget (or to `slots` /
+    `cells` when given, within the budget).  Worker threads generate the
+    scenes of arrived requests and build their psdata while the device
+    steps.  Each round of the loop admits into one pool every prepared
+    request that fits (in arrival order, looking past ones that do not fit
+    yet) and enqueues `graph_steps` batch steps for it, then waits for the
+    round enqueued before on the next pool and reads back and releases the
+    scenes that finished there.  With two pools (the default) the second
+    takes requests while the first is full: under a saturated queue the
+    device then runs one pool's round while the host serves the other and
+    does not wait for the host between rounds, and a light load runs in one
+    pool as with `pools=1`.  Profiling uses one pool.
     on_result(request, {field: array}) receives each result."""
     from .batch import Batch
 
+    npool = 1 if profile else max(1, int(pools))
     slots = slots or 32 << 20
     cells = cells or 2 * slots
-    while Batch.device_bytes(slots, cells, scene_entries) > hbm_budget and slots > 1024:
+    while npool * Batch.device_bytes(slots, cells, scene_entries) > hbm_budget and slots > 1024:
         slots //= 2
         cells //= 2
-    pool_bytes = Batch.device_bytes(slots, cells, scene_entries)
-    batch = Batch(slots, cells, scene_entries, compensated_sums=compensated_sums, polar_fp64=polar_fp64,
-                  graph_steps=0 if profile else graph_steps)
+    pool_bytes = npool * Batch.device_bytes(slots, cells, scene_entries)
+    batches = [Batch(slots, cells, scene_entries, compensated_sums=compensated_sums, polar_fp64=polar_fp64,
+                     graph_steps=0 if profile else graph_steps) for _ in range(npool)]
     prep = _Preparer(prepare_threads, prepare_ahead)
     if profile:
-        batch.set_profiling(True)
+        batches[0].set_profiling(True)
     stats = {}
     chunk = max(graph_steps, 1)
     pending = deque(sorted(requests, key=lambda r: r.arrival))
-    waiting = []                  # arrived, not admitted, in arrival order
-    running = {}                  # handle -> request
+    waiting = []                          # arrived, not admitted, in arrival order
+    running = [{} for _ in range(npool)]  # per pool: handle -> request
+    in_flight = [False] * npool           # a round enqueued and not yet collected
     done = {}
     busy_steps = 0
     peak_slots = peak_scenes = 0
@@ -150,86 +158,113 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
         for req in waiting:
             prep.submit(req)
 
-    try:
-        while pending or waiting or running:
-            arrivals()
-            # admission: arrival order, looking past requests that do not fit
-            # yet; stop at the first request still being prepared
-            admitted = set()
-            misses = 0
-            for req in waiting:
-                fut = prep.futures.get(req.index)
-                if fut is None:
+    full = [False] * npool                # the pool's last admission met a request that did not fit
+
+    def admit(k):
+        """Admission into pool k: arrival order, looking past requests that do
+        not fit yet; stops at the first request still being prepared.  A pool
+        after the first admits only while the one before it is full, so a
+        light load stays in one pool (one pool's round is the shortest)."""
+        nonlocal waiting
+        if k > 0 and not full[k - 1]:
+            return
+        batch = batches[k]
+        admitted = set()
+        misses = 0
+        for req in waiting:
+            fut = prep.futures.get(req.index)
+            if fut is None:
+                break
+            if not fut.done():
+                if any(running) or admitted:
                     break
-                if not fut.done():
-                    if running or admitted:
-                        break
-                    tp = time.perf_counter()
-                    concurrent.futures.wait([fut])       # nothing to step: wait for the scene
-                    phase["prepare_wait"] += time.perf_counter() - tp
-                prepared = fut.result()
-                if not batch.can_admit(prepared[1]):
-                    misses += 1
-                    if misses >= 4:        # the pool is close to full
-                        break
-                    continue
-                ta = time.perf_counter()
-                handle = batch.add_prepared(prepared, req.steps)
-                phase["admit"] += time.perf_counter() - ta
-                if handle is None:
-                    continue
-                del prep.futures[req.index]
-                running[handle] = req
-                admitted.add(req.index)
-            if admitted:
-                waiting = [r for r in waiting if r.index not in admitted]
-            if not running:
-                if waiting and waiting[0].index in prep.futures and prep.futures[waiting[0].index].done():
+                tp = time.perf_counter()
+                concurrent.futures.wait([fut])       # nothing to step: wait for the scene
+                phase["prepare_wait"] += time.perf_counter() - tp
+            prepared = fut.result()
+            if not batch.can_admit(prepared[1]):
+                if not running[k] and not admitted:
                     from .bindings import SphError
-                    raise SphError(f"request {waiting[0].index} does not fit the empty pool "
+                    raise SphError(f"request {req.index} does not fit the empty pool "
                                    f"({slots} slots, {cells} cells)")
+                misses += 1
+                if misses >= 4:        # the pool is close to full
+                    break
+                continue
+            ta = time.perf_counter()
+            handle = batch.add_prepared(prepared, req.steps)
+            phase["admit"] += time.perf_counter() - ta
+            if handle is None:
+                continue
+            del prep.futures[req.index]
+            running[k][handle] = req
+            admitted.add(req.index)
+        full[k] = misses > 0
+        if admitted:
+            waiting = [r for r in waiting if r.index not in admitted]
+
+    def collect(k):
+        """Wait for pool k's round; read back and release its finished scenes."""
+        if not in_flight[k]:
+            return
+        batch = batches[k]
+        tw = time.perf_counter()
+        batch.synchronize()
+        phase["wait"] += time.perf_counter() - tw
+        in_flight[k] = False
+        tr = time.perf_counter()
+        for handle in [h for h in running[k] if batch.steps_left(h) == 0]:
+            req = running[k].pop(handle)
+            result = batch.read(handle, *fields)
+            batch.remove(handle)
+            t_done = time.perf_counter() - t0
+            done[req.index] = {"latency": t_done - (req.arrival if realtime else 0.0), "finished": t_done}
+            if on_result:
+                on_result(req, result)
+        phase["read"] += time.perf_counter() - tr
+
+    try:
+        k = 0
+        while pending or waiting or any(running):
+            arrivals()
+            admit(k)
+            if running[k]:
+                infos = [b.info() for b in batches]
+                peak_slots = max(peak_slots, sum(i["used_slots"] for i in infos))
+                peak_scenes = max(peak_scenes, sum(i["scenes"] for i in infos))
+                te = time.perf_counter()
+                batches[k].step(chunk)
+                in_flight[k] = True
+                busy_steps += chunk
+                phase["enqueue"] += time.perf_counter() - te
+                arrivals()                     # preparation continues on the workers
+            nxt = (k + 1) % npool
+            collect(nxt)
+            if not any(running) and not any(in_flight):
                 if pending and not waiting:
                     ti = time.perf_counter()
                     time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.05)))
                     phase["idle"] += time.perf_counter() - ti
-                continue
-            info = batch.info()
-            peak_slots = max(peak_slots, info["used_slots"])
-            peak_scenes = max(peak_scenes, info["scenes"])
-            te = time.perf_counter()
-            batch.step(chunk)
-            busy_steps += chunk
-            phase["enqueue"] += time.perf_counter() - te
-            arrivals()                     # preparation continues on the workers
-            tw = time.perf_counter()
-            batch.synchronize()
-            phase["wait"] += time.perf_counter() - tw
-            tr = time.perf_counter()
-            for handle in [h for h in running if batch.steps_left(h) == 0]:
-                req = running.pop(handle)
-                result = batch.read(handle, *fields)
-                batch.remove(handle)
-                t_done = time.perf_counter() - t0
-                done[req.index] = {"latency": t_done - (req.arrival if realtime else 0.0), "finished": t_done}
-                if on_result:
-                    on_result(req, result)
-            phase["read"] += time.perf_counter() - tr
-            if log and busy_steps % (chunk * 100) == 0:
-                log(f"batched: {len(done)}/{len(requests)} done, {len(running)} resident, "
+            k = nxt
+            if log and busy_steps and busy_steps % (chunk * 100) == 0:
+                log(f"batched: {len(done)}/{len(requests)} done, {sum(len(r) for r in running)} resident, "
                     f"{len(waiting)} waiting, t={time.perf_counter() - t0:.1f}s")
+        for j in range(npool):
+            collect(j)
         t_end = time.perf_counter() - t0
-        info = batch.info()
+        infos = [b.info() for b in batches]
         if profile:
-            stats = batch.kernel_stats()
+            stats = batches[0].kernel_stats()
     finally:
         prep.close()
-        batch.free()
+        for b in batches:
+            b.free()
     t_first = requests[0].arrival if realtime else 0.0
     return _summary("batched", requests, done, min(t_first, t_end), t_end,
-                    {"pool_bytes": pool_bytes, "pool_slots": slots, "pool_cells": cells,
+                    {"pool_bytes": pool_bytes, "pool_slots": slots, "pool_cells": cells, "pools": npool,
                      "peak_resident_slots": peak_slots, "peak_resident_scenes": peak_scenes,
-                     "batch_steps": busy_steps, "graph_captures": info["graph_captures"],
-                     "graph_launches": info["graph_launches"], "graph_steps": graph_steps,
+                     "batch_steps": busy_steps, "graph_captures": sum(i["graph_captures"] for i in infos),
+                     "graph_launches": sum(i["graph_launches"] for i in infos), "graph_steps": graph_steps,
                      "kernel_stats": stats, "host_phases_s": {k: round(v, 4) for k, v in phase.items()}})
 
 
