@@ -346,7 +346,8 @@ def run_served(args):
         "config": {"workload": "served_stream", "requests": len(reqs), "rate_per_s": args.rate,
                    "mix": "50% fluid / 30% corotational / 20% svk", "particles": "log-uniform 1024-262144",
                    "steps_per_request": steps, "hbm_budget_gb": args.hbm_budget_gb, "sums": args.sums,
-                   "polar": "fp64", "graph_steps": args.graph_steps, "path": "batched", "tuning": tuning},
+                   "polar": "fp64", "graph_steps": args.graph_steps, "pools": sat.get("pools", 1), "path": "batched",
+                   "tuning": tuning},
         "batched_saturated": _brief(sat),
         "prefix": {"requests": len(prefix), "batched": _brief(sat_prefix), "plain": _brief(plain_prefix),
                    "speedup": round(sat_prefix["particle_steps_per_s"] / plain_prefix["particle_steps_per_s"], 3)},
