@@ -215,6 +215,12 @@ int global_sums_hip(psdata_hip *pso, double *out);
  * graph keep the size they were captured with. */
 int sph_hip_set_tuning(unsigned neighbour_block);
 int sph_hip_get_tuning(unsigned *neighbour_block);
+/* No counterpart in the reference: list entries whose neighbour loads the
+ * fluid force pass issues together (8, 12 or 16), process-wide; more loads
+ * in flight against fewer waves per SIMD.  Chosen by the autotuner.  No
+ * value changes a result bit (the summation order is the list order). */
+int sph_hip_set_fluid_pair_group(unsigned entries);
+int sph_hip_get_fluid_pair_group(unsigned *entries);
 /* Name, architecture and compute-unit count of the selected device (the key
  * of the autotuner's table). */
 int sph_hip_device_name(char *buf, unsigned len);

@@ -88,6 +88,10 @@ struct FieldPtrs {
 constexpr unsigned kMaxNeighbourBlock = 256;
 unsigned neighbour_block();
 bool set_neighbour_block(unsigned threads);
+// List entries whose neighbour loads the fluid force pass issues together
+// (8, 12 or 16): more loads in flight against fewer waves per SIMD.
+unsigned fluid_pair_group();
+bool set_fluid_pair_group(unsigned entries);
 
 // ---- neighbour grid (kernels_grid.hip) ------------------------------------
 struct BinScratch {

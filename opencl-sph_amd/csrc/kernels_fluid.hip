@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
 // compute_forces_fluids over the pair cache: the in-range list of each slot
 // and the pressures of the density pass (identical values to evaluating the
 // equation of state per pair).  Overflowed lists fall back to the scan.
-template <class PS, bool KAHAN>
+template <class PS, bool KAHAN, unsigned G>
 __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtrs f, PairCache c) {
     const unsigned i = ps.slot_xcd(blockDim.x);
     if (!ps.active(i)) return;
@@ -464,7 +464,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
             V3 x, v;
             float rho, p;
         };
-        for_each_listed<kPairGroup>(
+        for_each_listed<G>(
             c, i, cnt, [&](unsigned j) { return Nbr{load3(pos, j), load3(vel, j), load1(rho, j), load1(prs, j)}; },
             [&](unsigned j, const Nbr &n) {
                 if (j != i) acc.pair(visc, kc, xi, vi, rhoi, pi, n.x, n.v, n.rho, n.p);
@@ -614,8 +614,20 @@ hipError_t forces_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, i
     if (ps.slots() == 0) return hipSuccess;
     const unsigned nb = neighbour_block();
     const dim3 g = nbr_grid(ps.slots(), nb), b(nb);
-    if (compensated & kKahanForces) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true>), g, b, 0, s, ps, f, c);
-    else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false>), g, b, 0, s, ps, f, c);
+    const bool kahan = (compensated & kKahanForces) != 0;
+    switch (fluid_pair_group()) {
+    case 12:
+        if (kahan) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true, 12>), g, b, 0, s, ps, f, c);
+        else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false, 12>), g, b, 0, s, ps, f, c);
+        break;
+    case 16:
+        if (kahan) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true, 16>), g, b, 0, s, ps, f, c);
+        else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false, 16>), g, b, 0, s, ps, f, c);
+        break;
+    default:
+        if (kahan) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true, kPairGroup>), g, b, 0, s, ps, f, c);
+        else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false, kPairGroup>), g, b, 0, s, ps, f, c);
+    }
     return hipGetLastError();
 }
 

@@ -612,11 +612,18 @@ bool same_psdata(const psdata &a, const psdata &b) { return a.data == b.data && 
 namespace sphk {
 namespace {
 unsigned g_neighbour_block = kMaxNeighbourBlock;
+unsigned g_fluid_pair_group = 8;
 }
 unsigned neighbour_block() { return g_neighbour_block; }
 bool set_neighbour_block(unsigned threads) {
     if (threads != 64 && threads != 128 && threads != 256) return false;
     g_neighbour_block = threads;
+    return true;
+}
+unsigned fluid_pair_group() { return g_fluid_pair_group; }
+bool set_fluid_pair_group(unsigned entries) {
+    if (entries != 8 && entries != 12 && entries != 16) return false;
+    g_fluid_pair_group = entries;
     return true;
 }
 }  // namespace sphk
@@ -642,6 +649,18 @@ extern "C" int sph_hip_device_name(char *buf, unsigned len) {
 extern "C" int sph_hip_get_tuning(unsigned *neighbour_block) {
     if (!neighbour_block) return fail("null argument");
     *neighbour_block = sphk::neighbour_block();
+    return 0;
+}
+
+extern "C" int sph_hip_set_fluid_pair_group(unsigned entries) {
+    if (!sphk::set_fluid_pair_group(entries))
+        return fail("fluid pair group must be 8, 12 or 16 (got %u)", entries);
+    return 0;
+}
+
+extern "C" int sph_hip_get_fluid_pair_group(unsigned *entries) {
+    if (!entries) return fail("null argument");
+    *entries = sphk::fluid_pair_group();
     return 0;
 }
 

@@ -5,6 +5,8 @@ Search space:
     neighbour_block   threads per block of the neighbour-list passes and
                       candidate scans: 64, 128, 256
     graph_steps       batch steps per hipGraph (batches only): 2, 4, 8, 16
+    pair_group        list entries per load group of the fluid force pass:
+                      8, 12, 16 (tuned after the other two, with their winners)
 
 The cell size is not a tuning parameter: it is the reference's grid
 (resolution |hi - lo| / h per axis), which fixes the neighbour visiting
@@ -34,7 +36,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PACKAGED = os.path.join(HERE, "tuning.json")
 BLOCKS = (64, 128, 256)
 GRAPH_STEPS = (2, 4, 8, 16)
-DEFAULTS = {"neighbour_block": 256, "graph_steps": 8}
+PAIR_GROUPS = (8, 12, 16)
+DEFAULTS = {"neighbour_block": 256, "graph_steps": 8, "pair_group": 8}
 
 
 def user_table_path():
@@ -61,7 +64,7 @@ def lookup(signature, device=None):
 
 
 def apply(params):
-    bnd.set_tuning(params["neighbour_block"])
+    bnd.set_tuning(params["neighbour_block"], params.get("pair_group", DEFAULTS["pair_group"]))
 
 
 def _store(signature, params, timings, device):
@@ -90,26 +93,40 @@ def single_signature(scene):
     return f"single:{scene.system}:{scene.n}"
 
 
-def tune_single(scene, steps=10, repeats=3, blocks=BLOCKS, store=True, log=None):
-    """Neighbour block size for one scene on the single-scene path."""
+def tune_single(scene, steps=10, repeats=3, blocks=BLOCKS, pair_groups=PAIR_GROUPS, store=True, log=None):
+    """Neighbour block size (and, for fluids, the force pass's load group)
+    for one scene on the single-scene path."""
     from . import scenes
     system = scenes.build(scene)
     handle = system.psdata_opencl
     timings = {}
+
+    def run():
+        system.run(steps)
+        handle.synchronize()
     try:
         for nb in blocks:
-            bnd.set_tuning(nb)
-
-            def run():
-                system.run(steps)
-                handle.synchronize()
+            bnd.set_tuning(nb, DEFAULTS["pair_group"])
             timings[str(nb)] = _median_time(run, repeats)
             if log:
                 log(f"tune {single_signature(scene)}: block {nb}: {timings[str(nb)] / steps * 1e3:.3f} ms/step")
+        best_nb = int(min(timings, key=timings.get))
+        best_pg = DEFAULTS["pair_group"]
+        if scene.system == "fluid":
+            group_t = {DEFAULTS["pair_group"]: timings[str(best_nb)]}
+            for pg in pair_groups:
+                if pg in group_t:
+                    continue
+                bnd.set_tuning(best_nb, pg)
+                group_t[pg] = _median_time(run, repeats)
+                timings[f"{best_nb}/g{pg}"] = group_t[pg]
+                if log:
+                    log(f"tune {single_signature(scene)}: block {best_nb}, pair group {pg}: "
+                        f"{group_t[pg] / steps * 1e3:.3f} ms/step")
+            best_pg = min(group_t, key=group_t.get)
     finally:
         system.free()
-    best = min(timings, key=timings.get)
-    params = dict(DEFAULTS, neighbour_block=int(best))
+    params = dict(DEFAULTS, neighbour_block=best_nb, pair_group=int(best_pg))
     apply(params)
     if store:
         _store(single_signature(scene), params, timings, bnd.device_name())
@@ -120,8 +137,8 @@ def batch_signature(mix="50/30/20"):
     return f"batch:{mix}"
 
 
-def tune_batch(requests, blocks=BLOCKS, graph_steps=GRAPH_STEPS, repeats=2, compensated_sums=1, store=True,
-               signature=None, log=None):
+def tune_batch(requests, blocks=BLOCKS, graph_steps=GRAPH_STEPS, pair_groups=PAIR_GROUPS, repeats=2,
+               compensated_sums=1, store=True, signature=None, log=None):
     """Neighbour block size and steps per graph for a batch serving
     `requests` saturated (all queued at once)."""
     from . import serve
@@ -130,19 +147,30 @@ def tune_batch(requests, blocks=BLOCKS, graph_steps=GRAPH_STEPS, repeats=2, comp
     # own wall time, from the first admission to the last result
     total = sum(r.n for r in requests)
     slots = max(1 << 16, 2 * total)
+    def runner(k):
+        def run():
+            return serve.serve_batched(requests, realtime=False, graph_steps=k, slots=slots, cells=8 * slots,
+                                       compensated_sums=compensated_sums)["wall_s"]
+        return run
     for nb in blocks:
-        bnd.set_tuning(nb)
+        bnd.set_tuning(nb, DEFAULTS["pair_group"])
         for k in graph_steps:
-            def run():
-                return serve.serve_batched(requests, realtime=False, graph_steps=k, slots=slots, cells=8 * slots,
-                                           compensated_sums=compensated_sums)["wall_s"]
             key = f"{nb}/{k}"
-            timings[key] = _median_time(run, repeats)
+            timings[key] = _median_time(runner(k), repeats)
             if log:
                 log(f"tune batch: block {nb}, graph steps {k}: {timings[key]:.3f} s")
     best = min(timings, key=timings.get)
     nb, k = (int(x) for x in best.split("/"))
-    params = dict(DEFAULTS, neighbour_block=nb, graph_steps=k)
+    group_t = {DEFAULTS["pair_group"]: timings[best]}
+    for pg in pair_groups:
+        if pg in group_t:
+            continue
+        bnd.set_tuning(nb, pg)
+        group_t[pg] = _median_time(runner(k), repeats)
+        timings[f"{nb}/{k}/g{pg}"] = group_t[pg]
+        if log:
+            log(f"tune batch: block {nb}, graph steps {k}, pair group {pg}: {group_t[pg]:.3f} s")
+    params = dict(DEFAULTS, neighbour_block=nb, graph_steps=k, pair_group=int(min(group_t, key=group_t.get)))
     apply(params)
     if store:
         _store(signature or batch_signature(), params, timings, bnd.device_name())

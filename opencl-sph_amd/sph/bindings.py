@@ -140,6 +140,8 @@ def _names_array(names):
 
 _set_tuning = _fn("sph_hip_set_tuning", [c_uint], c_int)
 _get_tuning = _fn("sph_hip_get_tuning", [POINTER(c_uint)], c_int)
+_set_pair_group = _fn("sph_hip_set_fluid_pair_group", [c_uint], c_int)
+_get_pair_group = _fn("sph_hip_get_fluid_pair_group", [POINTER(c_uint)], c_int)
 _device_name = _fn("sph_hip_device_name", [ctypes.c_char_p, c_uint], c_int)
 
 
@@ -147,14 +149,19 @@ def init_hip():
     check(_init_hip())
 
 
-def set_tuning(neighbour_block):
+def set_tuning(neighbour_block, pair_group=None):
+    """Threads per block of the neighbour passes; list entries per load group
+    of the fluid force pass (unchanged when None)."""
     check(_set_tuning(int(neighbour_block)))
+    if pair_group is not None:
+        check(_set_pair_group(int(pair_group)))
 
 
 def get_tuning():
-    out = c_uint()
+    out, pg = c_uint(), c_uint()
     check(_get_tuning(ctypes.byref(out)))
-    return {"neighbour_block": out.value}
+    check(_get_pair_group(ctypes.byref(pg)))
+    return {"neighbour_block": out.value, "pair_group": pg.value}
 
 
 def device_name():

@@ -95,18 +95,20 @@ def test_twenty_steps_drift(system):
 
 @pytest.mark.parametrize("system", ["fluid", "corotational", "svk"])
 def test_tuned_block_sizes_change_no_bit(system):
-    """Every neighbour block size the autotuner may pick gives the same bits."""
+    """Every neighbour block size and fluid force load group the autotuner
+    may pick gives the same bits."""
     from sph import bindings
     sc = small(system)
     fields = ("position", "velocity", "density", "force")
-    before = bindings.get_tuning()["neighbour_block"]
+    before = bindings.get_tuning()
+    settings = [(64, 8), (128, 8), (256, 8), (256, 12), (256, 16), (64, 16)]
     try:
         outs = {}
-        for nb in (64, 128, 256):
-            bindings.set_tuning(nb)
-            outs[nb] = run(sc, 5, 1, True, fields)[0]
+        for nb, pg in settings:
+            bindings.set_tuning(nb, pg)
+            outs[(nb, pg)] = run(sc, 5, 1, True, fields)[0]
     finally:
-        bindings.set_tuning(before)
-    for nb in (64, 128):
+        bindings.set_tuning(before["neighbour_block"], before["pair_group"])
+    for key in settings[1:]:
         for k in fields:
-            assert np.array_equal(outs[nb][k], outs[256][k]), (system, nb, k)
+            assert np.array_equal(outs[key][k], outs[settings[0]][k]), (system, key, k)
