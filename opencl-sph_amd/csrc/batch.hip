@@ -119,13 +119,6 @@ __global__ void __launch_bounds__(kBlock) k_batch_scatter(SceneTable ps, const u
     tmp_key[t] = k;
 }
 
-template <unsigned W, typename T>
-__device__ __forceinline__ void copy_words(const T *__restrict__ src, T *__restrict__ dst, unsigned from,
-                                           unsigned to) {
-#pragma unroll
-    for (unsigned w = 0; w < W; ++w) dst[static_cast<size_t>(to) * W + w] = src[static_cast<size_t>(from) * W + w];
-}
-
 // Stable placement (ascending particle id inside each cell) and the move of
 // the persistent state of each particle into its new slot.  The unsorted
 // positions of a scene's particles after the scatter are exactly its slots
@@ -145,9 +138,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_rank_gather(SceneTable ps, con
         const unsigned c = tmp_key[t];
         const unsigned start = celloffset[c], cnt = gridcount[c];
         const unsigned me = tmp_id[t];
-        unsigned rank = 0;
-        for (unsigned u = start; u < start + cnt; ++u) rank += tmp_id[u] < me ? 1u : 0u;
-        f = start + rank;
+        f = start + rank_in_cell(tmp_id, start, cnt, me);
         from = tmp_src[t];
         gridcell[f] = c;
     }

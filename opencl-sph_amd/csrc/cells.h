@@ -52,4 +52,49 @@ __device__ __forceinline__ unsigned claim_ranks(unsigned k, bool active, unsigne
     return base + static_cast<unsigned>(lane - start);
 }
 
+// Rank of id `me` among the ids of its cell, ids[start, start + cnt): the
+// number of smaller ids, which is me's offset in the stable (ascending id)
+// order of the cell.  The ids are loaded 8 at a time, unconditionally (slots
+// past the cell's end re-read its last member and are not counted), so a cell
+// costs one memory round trip per 8 members rather than one per member.  The
+// cell holds me itself, so cnt >= 1.
+__device__ __forceinline__ unsigned rank_in_cell(const unsigned *__restrict__ ids, unsigned start, unsigned cnt,
+                                                 unsigned me) {
+    constexpr unsigned kBatch = 8;
+    const unsigned last = start + cnt - 1u;
+    unsigned rank = 0;
+    for (unsigned k0 = 0; k0 < cnt; k0 += kBatch) {
+        unsigned v[kBatch];
+#pragma unroll
+        for (unsigned k = 0; k < kBatch; ++k) v[k] = ids[min(start + k0 + k, last)];
+#pragma unroll
+        for (unsigned k = 0; k < kBatch; ++k) rank += (k0 + k < cnt && v[k] < me) ? 1u : 0u;
+    }
+    return rank;
+}
+
+// W consecutive 32-bit words with 4-byte alignment, moved as one wide load
+// and one wide store (global_load_dwordx3 for a position, and so on).
+template <unsigned W>
+struct __attribute__((packed, aligned(4))) Words {
+    unsigned w[W];
+};
+// Record `rec` of an array of W-word records.
+template <unsigned W, typename T>
+__device__ __forceinline__ Words<W> load_words(const T *__restrict__ p, unsigned rec) {
+    static_assert(sizeof(T) == 4, "records of 32-bit words");
+    return *reinterpret_cast<const Words<W> *>(p + static_cast<size_t>(rec) * W);
+}
+template <unsigned W, typename T>
+__device__ __forceinline__ void store_words(T *__restrict__ p, unsigned rec, const Words<W> &v) {
+    static_assert(sizeof(T) == 4, "records of 32-bit words");
+    *reinterpret_cast<Words<W> *>(p + static_cast<size_t>(rec) * W) = v;
+}
+// dst record `to` = src record `from`.
+template <unsigned W, typename T>
+__device__ __forceinline__ void copy_words(const T *__restrict__ src, T *__restrict__ dst, unsigned from,
+                                           unsigned to) {
+    store_words<W>(dst, to, load_words<W>(src, from));
+}
+
 }  // namespace sphk

@@ -57,7 +57,22 @@ __device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y -
 __device__ __forceinline__ V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
 __device__ __forceinline__ V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
 __device__ __forceinline__ V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ V3 operator/(V3 a, float s) { return V3{a.x / s, a.y / s, a.z / s}; }
+// Float division through a double reciprocal: float(double(a) * (1.0 / double(b)))
+// is the correctly rounded a / b for every pair of floats (zeros, infinities,
+// NaNs and subnormals included).  The exact quotient of two 24-bit
+// significands is never a float midpoint and lies at least 2^-49 (relative)
+// from every midpoint, in the normal and the subnormal range; the double
+// product is within 2^-52 of the quotient, so rounding it to float cannot
+// cross a midpoint.  Several quotients by one divisor then cost one double
+// division and a multiply each instead of one float division each.
+__device__ __forceinline__ double recip(float b) { return 1.0 / static_cast<double>(b); }
+__device__ __forceinline__ float div_recip(float a, double rb) {
+    return static_cast<float>(static_cast<double>(a) * rb);
+}
+__device__ __forceinline__ V3 div_recip(V3 a, double rb) {
+    return V3{div_recip(a.x, rb), div_recip(a.y, rb), div_recip(a.z, rb)};
+}
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return div_recip(a, recip(s)); }
 __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ float sqlength(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
 __device__ __forceinline__ float length(V3 a) { return sqrtf(sqlength(a)); }

@@ -400,9 +400,10 @@ struct FluidForce {
         const float pterm = static_cast<float>(static_cast<double>(pi + pj) * 0.5 / static_cast<double>(r) *
                                                static_cast<double>(din) * static_cast<double>(din) *
                                                static_cast<double>(kc.spiky));
-        fx.add((d.x * pterm + rv.x * vterm) / didj);
-        fy.add((d.y * pterm + rv.y * vterm) / didj);
-        fz.add((d.z * pterm + rv.z * vterm) / didj);
+        const double rdidj = recip(didj);   // the three quotients below, bit for bit
+        fx.add(div_recip(d.x * pterm + rv.x * vterm, rdidj));
+        fy.add(div_recip(d.y * pterm + rv.y * vterm, rdidj));
+        fz.add(div_recip(d.z * pterm + rv.z * vterm, rdidj));
     }
     __device__ __forceinline__ V3 finish(const StepParams &p, V3 xi) const {
         V3 fi = V3{fx.value(), fy.value(), fz.value()} * p.mass;

@@ -137,6 +137,8 @@ __global__ void k_scatter_unsorted(unsigned n, const unsigned *__restrict__ key,
     tmp_key[t] = k;
 }
 
+// Moves the position (v0) and, with VEL, the velocity (v1) with the particle.
+template <bool VEL>
 __global__ void k_rank_place(unsigned n, const unsigned *__restrict__ tmp_id,
                              const unsigned *__restrict__ tmp_src, const unsigned *__restrict__ tmp_key,
                              const unsigned *__restrict__ celloffset,
@@ -148,26 +150,21 @@ __global__ void k_rank_place(unsigned n, const unsigned *__restrict__ tmp_id,
     unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const unsigned c = tmp_key[t];
-    const unsigned start = celloffset[c], cnt = gridcount[c];
     const unsigned me = tmp_id[t];
-    unsigned rank = 0;
-    for (unsigned u = start; u < start + cnt; ++u) rank += tmp_id[u] < me ? 1u : 0u;
-    const unsigned f = start + rank;
     const unsigned src = tmp_src[t];
+    // the moved vectors are fetched first, so that their latency overlaps
+    // the rank count
+    const Words<3> w0 = load_words<3>(v0s, src);
+    Words<3> w1{};
+    if (VEL) w1 = load_words<3>(v1s, src);
+    const unsigned start = celloffset[c], cnt = gridcount[c];
+    const unsigned f = start + rank_in_cell(tmp_id, start, cnt, me);
     perm[f] = src;
     slot_id_new[f] = me;
     gridcell[f] = c;
     cellparticles[f] = me;
-    if (v0d) {
-        v0d[3 * f] = v0s[3 * src];
-        v0d[3 * f + 1] = v0s[3 * src + 1];
-        v0d[3 * f + 2] = v0s[3 * src + 2];
-    }
-    if (v1d) {
-        v1d[3 * f] = v1s[3 * src];
-        v1d[3 * f + 1] = v1s[3 * src + 1];
-        v1d[3 * f + 2] = v1s[3 * src + 2];
-    }
+    store_words<3>(v0d, f, w0);
+    if (VEL) store_words<3>(v1d, f, w1);
 }
 
 template <unsigned W>
@@ -271,9 +268,15 @@ hipError_t launch_rank_place(unsigned n, const BinScratch &b, const unsigned *ce
                              unsigned *gridcell, unsigned *cellparticles, const float *vec_src0,
                              float *vec_dst0, const float *vec_src1, float *vec_dst1, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rank_place, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, b.tmp_id, b.tmp_src,
-                       b.tmp_key, celloffset, gridcount, perm, slot_id_new, gridcell, cellparticles,
-                       vec_src0, vec_dst0, vec_src1, vec_dst1);
+    if (!vec_src0 || !vec_dst0) return hipErrorInvalidValue;
+    if (vec_src1 && vec_dst1)
+        hipLaunchKernelGGL(k_rank_place<true>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, b.tmp_id, b.tmp_src,
+                           b.tmp_key, celloffset, gridcount, perm, slot_id_new, gridcell, cellparticles,
+                           vec_src0, vec_dst0, vec_src1, vec_dst1);
+    else
+        hipLaunchKernelGGL(k_rank_place<false>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, b.tmp_id, b.tmp_src,
+                           b.tmp_key, celloffset, gridcount, perm, slot_id_new, gridcell, cellparticles,
+                           vec_src0, vec_dst0, nullptr, nullptr);
     return hipGetLastError();
 }
 

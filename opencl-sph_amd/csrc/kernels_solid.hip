@@ -197,6 +197,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
     const float rt[9] = {rot[0], rot[3], rot[6], rot[1], rot[4], rot[7], rot[2], rot[5], rot[8]};
 
     const float d0i = f.density0[i];
+    const double rd0i = recip(d0i);   // the quotients (u * mass) / d0i below, bit for bit
     const float mass = p.mass;   // read once (see k_forces_solids)
     Acc<KAHAN> du[9];
     if (MODE == kListedOnly) {
@@ -204,7 +205,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
             if (!(e & kListReference) || (e & kListSlot) == i) return;
             const V3 q = d.x0 - x0i;
             const V3 u = mat_mul(rt, d.x - xi) - q;
-            add_outer(du, (u * mass) / d0i, grad_spiky(-q, kc));
+            add_outer(du, div_recip(u * mass, rd0i), grad_spiky(-q, kc));
         });
     } else for_each_candidate(p.grid, p.p0, p.p0 + p.n, key, f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
@@ -212,7 +213,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
             const V3 q = x0j - x0i;
             if (!(length(q) > kc.h)) {
                 const V3 u = mat_mul(rt, xj - xi) - q;
-                add_outer(du, (u * mass) / d0i, grad_spiky(-q, kc));
+                add_outer(du, div_recip(u * mass, rd0i), grad_spiky(-q, kc));
             }
         }
         if (inear) {
@@ -223,7 +224,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
                     const V3 pv = mat_mul(rt, reflect_combo(p, b, xj) - xi);
                     const V3 q = reflect_combo(p, b, x0j) - x0i;
                     const V3 u = pv - q;
-                    if (length(pv) < kc.h) add_outer(du, (u * mass) / d0i, grad_spiky(-q, kc));
+                    if (length(pv) < kc.h) add_outer(du, div_recip(u * mass, rd0i), grad_spiky(-q, kc));
                 }
         }
     });
