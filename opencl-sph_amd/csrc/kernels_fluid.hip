@@ -141,6 +141,26 @@ __device__ __forceinline__ bool row_range(const GridGeom &g, unsigned lo, unsign
     return true;
 }
 
+// First slots of the 9 neighbour rows of a lane, kept in registers between
+// the scan and the kernel sum: the scan pushes them in row order (every row,
+// inside the grid or not), the sum picks the row of each list entry with a
+// select tree instead of reloading celloffset.
+struct RowBases {
+    unsigned b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    __device__ __forceinline__ void push(unsigned v) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) b[k] = b[k + 1];
+        b[8] = v;
+    }
+    __device__ __forceinline__ unsigned pick(unsigned r) const {   // r in 0..8
+        const bool o1 = r & 1u, o2 = r & 2u, o4 = r & 4u;
+        const unsigned a01 = o1 ? b[1] : b[0], a23 = o1 ? b[3] : b[2];
+        const unsigned a45 = o1 ? b[5] : b[4], a67 = o1 ? b[7] : b[6];
+        const unsigned a03 = o2 ? a23 : a01, a47 = o2 ? a67 : a45;
+        return r >= 8u ? b[8] : (o4 ? a47 : a03);
+    }
+};
+
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs f, PairCache c) {
     __shared__ unsigned short buf[kListCap * kListBlock];
@@ -158,34 +178,37 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     unsigned short *col = buf + threadIdx.x;
     unsigned cnt = 0;
     bool fast = !inear && key >= g.c0 && key - g.c0 < g.ncells;
-    unsigned xo = 0, yo = 0, zo = 0;
+    RowBases rows;
     if (fast) {
         const unsigned local = key - g.c0;
-        zo = local / g.layer;
+        const unsigned zo = local / g.layer;
         const unsigned rem = local - g.layer * zo;
-        yo = rem / g.rx;
-        xo = rem - g.rx * yo;
+        const unsigned yo = rem / g.rx;
+        const unsigned xo = rem - g.rx * yo;
 #pragma unroll 1
         for (unsigned r = 0; r < 9u; ++r) {
-            unsigned b, e;
-            if (!row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e)) continue;
+            unsigned b = 0, e = 0;
+            const bool inside = row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e);
+            rows.push(b);
+            if (!inside) continue;
             if (e - b > (1u << kRowBits)) fast = false;
             // kScanUnroll candidates per round, loads first (slots past the
             // row's end are read, within the buffer's kScanPad padding, and
-            // not used), tested in slot order
+            // not used), tested in slot order.  j0 - b is a multiple of
+            // kScanUnroll, so entry (r << kRowBits) | (j0 + u - b) is ent + u.
             static_assert(kScanUnroll <= kScanPad, "scan rounds read past the row end");
             for (unsigned j0 = b; j0 < e; j0 += kScanUnroll) {
                 V3 xs[kScanUnroll];
                 const unsigned off = j0 * 12u;
 #pragma unroll
                 for (unsigned u = 0; u < kScanUnroll; ++u) xs[u] = load3_at(pos, off + 12u * u);
+                const unsigned left = e - j0;   // candidates of this round inside the row
+                const unsigned ent = (r << kRowBits) | (j0 - b);
 #pragma unroll
                 for (unsigned u = 0; u < kScanUnroll; ++u) {
-                    if (j0 + u >= e) break;
                     const float s = sqlength(xs[u] - xi);
-                    if (!(s >= r2_out)) {
-                        if (cnt < kListCap)
-                            col[cnt * kListBlock] = static_cast<unsigned short>((r << kRowBits) | (j0 + u - b));
+                    if ((u < left) & !(s >= r2_out)) {
+                        if (cnt < kListCap) col[cnt * kListBlock] = static_cast<unsigned short>(ent + u);
                         ++cnt;
                     }
                 }
@@ -197,7 +220,6 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     unsigned nn = 0;
     if (fast) {
         Acc<KAHAN> acc;
-        unsigned cur = 0xffu, base = 0, e = 0;
         for (unsigned k0 = 0; k0 < cnt; k0 += kPairGroup) {
             unsigned js[kPairGroup];
 #pragma unroll
@@ -205,12 +227,7 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
                 js[q] = i;
                 if (k0 + q < cnt) {
                     const unsigned ent = col[(k0 + q) * kListBlock];
-                    const unsigned r = ent >> kRowBits;
-                    if (r != cur) {
-                        cur = r;
-                        row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &base, &e);
-                    }
-                    js[q] = base + (ent & ((1u << kRowBits) - 1u));
+                    js[q] = rows.pick(ent >> kRowBits) + (ent & ((1u << kRowBits) - 1u));
                 }
             }
             V3 xs[kPairGroup];
@@ -271,17 +288,19 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
     } else {
         const V3 x0i = load3(opos, i);
         bool fast = key >= g.c0 && key - g.c0 < g.ncells;
-        unsigned xo = 0, yo = 0, zo = 0;
+        RowBases rows;
         if (fast) {
             const unsigned local = key - g.c0;
-            zo = local / g.layer;
+            const unsigned zo = local / g.layer;
             const unsigned rem = local - g.layer * zo;
-            yo = rem / g.rx;
-            xo = rem - g.rx * yo;
+            const unsigned yo = rem / g.rx;
+            const unsigned xo = rem - g.rx * yo;
 #pragma unroll 1
             for (unsigned r = 0; r < 9u; ++r) {
-                unsigned b, e;
-                if (!row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e)) continue;
+                unsigned b = 0, e = 0;
+                const bool inside = row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e);
+                rows.push(b);
+                if (!inside) continue;
                 if (e - b > (1u << kSolidOffsetBits)) fast = false;
                 static_assert(kSolidScanUnroll <= kScanPad, "scan rounds read past the row end");
                 for (unsigned j0 = b; j0 < e; j0 += kSolidScanUnroll) {
@@ -292,15 +311,16 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                         xs[u] = load3_at(pos, off + 12u * u);
                         x0s[u] = load3_at(opos, off + 12u * u);
                     }
+                    const unsigned left = e - j0;   // candidates of this round inside the row
+                    const unsigned ent = (r << kRowBits) | (j0 - b);   // j0 - b: a multiple of the unroll
 #pragma unroll
                     for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
-                        if (j0 + u >= e) break;
                         const bool cur = !(sqlength(xs[u] - xi) >= r2_out);
                         const bool ref = !(sqlength(x0s[u] - x0i) >= r2_out);
-                        if (cur || ref) {
+                        if ((u < left) & (cur | ref)) {
                             if (cnt < kListCap)
                                 col[cnt * kListBlock] = static_cast<unsigned short>(
-                                    (r << kRowBits) | (cur ? 1u << 11 : 0u) | (ref ? 1u << 10 : 0u) | (j0 + u - b));
+                                    (ent + u) | (cur ? 1u << 11 : 0u) | (ref ? 1u << 10 : 0u));
                             ++cnt;
                         }
                     }
@@ -310,7 +330,6 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
         }
         Acc<KAHAN> acc;
         if (fast) {
-            unsigned cur_row = 0xffu, base = 0, e = 0;
             for (unsigned k0 = 0; k0 < cnt; k0 += kPairGroup) {
                 unsigned ents[kPairGroup], js[kPairGroup];
 #pragma unroll
@@ -319,12 +338,7 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                     js[q] = i;
                     if (k0 + q < cnt) {
                         ents[q] = col[(k0 + q) * kListBlock];
-                        const unsigned r = ents[q] >> kRowBits;
-                        if (r != cur_row) {
-                            cur_row = r;
-                            row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &base, &e);
-                        }
-                        js[q] = base + (ents[q] & ((1u << kSolidOffsetBits) - 1u));
+                        js[q] = rows.pick(ents[q] >> kRowBits) + (ents[q] & ((1u << kSolidOffsetBits) - 1u));
                     }
                 }
                 V3 xs[kPairGroup];
