@@ -157,6 +157,13 @@ __global__ void __launch_bounds__(kBlock) k_batch_rank_gather(SceneTable ps, con
     }
 }
 
+// Clears the setup flag of every scene once the setup step is enqueued (the
+// device copy of the scene table; the host mirror is cleared alongside).
+__global__ void k_batch_clear_setup(unsigned nscenes, StepParams *scenes) {
+    const unsigned s = linear_slot();
+    if (s < nscenes) scenes[s].flags &= ~kSceneSetup;
+}
+
 __global__ void k_batch_tick(unsigned nscenes, unsigned *steps_left) {
     const unsigned s = linear_slot();
     if (s < nscenes && steps_left[s] > 0u) steps_left[s] -= 1u;
@@ -880,15 +887,11 @@ extern "C" int sph_batch_step(sph_batch *b, unsigned num_steps) {
         if (b->used_slots == 0) break;
         if (b->setup_pending) {
             if (enqueue_step(b, systems_present(b), true)) return -1;
-            for (unsigned s = 0; s < b->scene_cap; ++s)
-                if (b->seg_of[s] != kNoScene && (b->scenes[s].flags & sphk::kSceneSetup)) {
-                    b->scenes[s].flags &= ~sphk::kSceneSetup;
-                    SPH_HIP_TRY(hipMemcpyAsync(&b->d_scenes[s].flags, &b->scenes[s].flags, 4, hipMemcpyHostToDevice,
-                                               b->stream));
-                }
+            for (unsigned s = 0; s < b->scene_cap; ++s) b->scenes[s].flags &= ~sphk::kSceneSetup;
+            hipLaunchKernelGGL(sphk::k_batch_clear_setup, sphk::blocks(b->scene_cap), dim3(sphk::kBlock), 0,
+                               b->stream, b->scene_cap, b->d_scenes);
+            SPH_HIP_TRY(hipGetLastError());
             b->setup_pending = 0;
-            // the upload reads host memory that the next step may change
-            SPH_HIP_TRY(hipStreamSynchronize(b->stream));
             done += 1;
         } else if (b->graph_steps > 0 && !b->profiling && num_steps - done >= b->graph_steps &&
                    (b->graph_version == b->layout_version || b->eager_version == b->layout_version)) {
