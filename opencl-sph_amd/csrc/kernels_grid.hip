@@ -76,13 +76,26 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned *t
     return wave_off + inc - v;
 }
 
+// The scan runs in tiles of kScanTile cells: per-tile sums, the exclusive
+// scan of the tile sums, then each tile's own scan.  VEC: the arrays are
+// 16-byte aligned and full tiles move as 16-byte words.
+template <bool VEC>
 __global__ void __launch_bounds__(kBlock) k_scan_reduce(const unsigned *__restrict__ in, unsigned count,
                                                         unsigned *__restrict__ tile_sums) {
     const unsigned base = blockIdx.x * kScanTile;
     unsigned sum = 0;
-    for (unsigned t = threadIdx.x; t < kScanTile; t += kBlock) {
-        unsigned i = base + t;
-        if (i < count) sum += in[i];
+    if (VEC && base + kScanTile <= count) {
+        const uint4 *v = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+        for (unsigned k = 0; k < kScanTile / 4 / kBlock; ++k) {
+            const uint4 q = v[k * kBlock + threadIdx.x];
+            sum += (q.x + q.y) + (q.z + q.w);
+        }
+    } else {
+        for (unsigned t = threadIdx.x; t < kScanTile; t += kBlock) {
+            unsigned i = base + t;
+            if (i < count) sum += in[i];
+        }
     }
     unsigned total;
     block_exclusive_scan(sum, &total);
@@ -101,25 +114,61 @@ __global__ void __launch_bounds__(kBlock) k_scan_tile_sums(unsigned *tile_sums, 
     }
 }
 
+// Tiles up to which each tile of the final pass sums the tile sums before it
+// itself (at most kPrefixTiles values per block) instead of a separate
+// one-block scan of the tile sums.
+constexpr unsigned kPrefixTiles = 4096;
+
+// PREFIX: tile_sums holds the raw tile sums and each block adds up those of
+// the tiles before it; otherwise tile_sums holds their exclusive scan.
+template <bool VEC, bool PREFIX>
 __global__ void __launch_bounds__(kBlock) k_scan_final(const unsigned *__restrict__ in, unsigned count,
-                                                       const unsigned *__restrict__ tile_off,
+                                                       const unsigned *__restrict__ tile_sums,
                                                        unsigned *__restrict__ out) {
+    unsigned tile_off;
+    if (PREFIX) {
+        unsigned pre = 0;
+        for (unsigned k = threadIdx.x; k < blockIdx.x; k += kBlock) pre += tile_sums[k];
+        block_exclusive_scan(pre, &tile_off);
+    } else {
+        tile_off = tile_sums[blockIdx.x];
+    }
     const unsigned base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    const bool full = VEC && base + kScanItems <= count;
     unsigned v[kScanItems];
+    if (full) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+        for (unsigned k = 0; k < kScanItems / 4; ++k) {
+            const uint4 q = src[k];
+            v[4 * k] = q.x;
+            v[4 * k + 1] = q.y;
+            v[4 * k + 2] = q.z;
+            v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (unsigned k = 0; k < kScanItems; ++k) v[k] = base + k < count ? in[base + k] : 0u;
+    }
     unsigned sum = 0;
 #pragma unroll
-    for (unsigned k = 0; k < kScanItems; ++k) {
-        unsigned i = base + k;
-        v[k] = i < count ? in[i] : 0u;
-        sum += v[k];
-    }
+    for (unsigned k = 0; k < kScanItems; ++k) sum += v[k];
     unsigned total;
-    unsigned run = tile_off[blockIdx.x] + block_exclusive_scan(sum, &total);
+    unsigned run = tile_off + block_exclusive_scan(sum, &total);
+    unsigned o[kScanItems];
 #pragma unroll
     for (unsigned k = 0; k < kScanItems; ++k) {
-        unsigned i = base + k;
-        if (i < count) out[i] = run;
+        o[k] = run;
         run += v[k];
+    }
+    if (full) {
+        uint4 *dst = reinterpret_cast<uint4 *>(out + base);
+#pragma unroll
+        for (unsigned k = 0; k < kScanItems / 4; ++k) dst[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (unsigned k = 0; k < kScanItems; ++k)
+            if (base + k < count) out[base + k] = o[k];
     }
 }
 
@@ -248,10 +297,19 @@ hipError_t launch_key_count(const float *position, unsigned n, const GridGeom &g
 hipError_t launch_exclusive_scan(const unsigned *in, unsigned count, unsigned *out, unsigned *tile_sums,
                                  hipStream_t s) {
     if (count == 0) return hipSuccess;
-    unsigned ntiles = (count + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kBlock), 0, s, in, count, tile_sums);
-    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kBlock), 0, s, tile_sums, ntiles);
-    hipLaunchKernelGGL(k_scan_final, dim3(ntiles), dim3(kBlock), 0, s, in, count, tile_sums, out);
+    const unsigned ntiles = (count + kScanTile - 1) / kScanTile;
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0;
+    const dim3 grid(ntiles), block(kBlock);
+    if (vec) hipLaunchKernelGGL(k_scan_reduce<true>, grid, block, 0, s, in, count, tile_sums);
+    else hipLaunchKernelGGL(k_scan_reduce<false>, grid, block, 0, s, in, count, tile_sums);
+    if (ntiles <= kPrefixTiles) {
+        if (vec) hipLaunchKernelGGL((k_scan_final<true, true>), grid, block, 0, s, in, count, tile_sums, out);
+        else hipLaunchKernelGGL((k_scan_final<false, true>), grid, block, 0, s, in, count, tile_sums, out);
+    } else {
+        hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), block, 0, s, tile_sums, ntiles);
+        if (vec) hipLaunchKernelGGL((k_scan_final<true, false>), grid, block, 0, s, in, count, tile_sums, out);
+        else hipLaunchKernelGGL((k_scan_final<false, false>), grid, block, 0, s, in, count, tile_sums, out);
+    }
     return hipGetLastError();
 }
 

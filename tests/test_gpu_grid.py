@@ -84,6 +84,31 @@ def test_bins_out_of_bounds_partial_and_ragged(hip_device):
     s.free()
 
 
+@pytest.mark.parametrize("h", [0.0093, 0.0038], ids=["300_tiles", "4442_tiles"])
+def test_bins_many_scan_tiles(hip_device, h):
+    """Cell scans of 1.23 M and 18.2 M cells (scan tiles of 4096 cells, the
+    last one partial): below and above the tile count (4096) up to which each
+    tile sums the tiles before it itself rather than after a separate scan of
+    the tile sums."""
+    rng = np.random.Generator(np.random.PCG64(13))
+    pnum = 6000
+    pos = rng.uniform(0.0, 1.0, size=(pnum, 3)).astype(np.float32)
+    pos[:300] = np.float32(0.2)              # one crowded cell
+    spec = scenes.fluid_spec(pnum, 0.008, 1e-4, h, (0, 1, 0, 1, 0, 1), 1000, 2000, 0.1)
+    scene = scenes.Scene("many_tiles", "fluid", spec, scenes.FLUID_KERNELS,
+                         {"position": pos, "n": np.array([pnum], np.uint32)},
+                         dict(n=pnum, gridbounds=[0, 1, 0, 1, 0, 1], h=h, mass=0.008, dt=1e-4))
+    res = int(np.float32(1.0) / np.float32(h))
+    say(f"bins on a {res}^3 grid ({res ** 3} cells), n={pnum}")
+    s = engine(scene)
+    s.compute_particle_bins()
+    st = oracle_state(scene, position=pos)
+    got = check_bins(s, st, pnum)
+    assert got["gridcount"].size == res ** 3
+    assert got["gridcount"].size % 4096 != 0
+    s.free()
+
+
 def test_bins_empty(hip_device):
     spec = scenes.fluid_spec(64, 0.008, 1e-4, 0.1, (0, 1, 0, 1, 0, 1), 1000, 2000, 0.1)
     scene = scenes.Scene("empty", "fluid", spec, scenes.FLUID_KERNELS,
