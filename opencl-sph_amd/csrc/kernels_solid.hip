@@ -507,11 +507,11 @@ inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 inline dim3 nbr_grid(unsigned n, unsigned block) { return dim3(xcd_grid((n + block - 1) / block)); }
 
 // Launches kernel template K<MODE> as the list-only and overflow-scan pair
-// when `c` holds a solid list, else as one scan over every lane.
+// when `c` holds a solid list, else as one scan over every lane, with `nb`
+// threads per block.
 template <template <int> class K, class PS>
-hipError_t launch_modes(const PS &ps, const FieldPtrs &f, const PairCache &c, hipStream_t s) {
+hipError_t launch_modes(const PS &ps, const FieldPtrs &f, const PairCache &c, hipStream_t s, unsigned nb) {
     if (ps.slots() == 0) return hipSuccess;
-    const unsigned nb = neighbour_block();
     const dim3 g = nbr_grid(ps.slots(), nb), b(nb);
     if (c.valid == kSolidList) {
         hipLaunchKernelGGL(K<kListedOnly>::template kernel<PS>, g, b, 0, s, ps, f, c);
@@ -547,14 +547,19 @@ struct ForcesFor {
     };
 };
 
+// The rotation/strain and deformation passes keep their per-lane 3x3 work
+// arrays in LDS, sized by the compiler for kBlock threads per block whatever
+// the launch size, so they always run kBlock-thread blocks: smaller blocks
+// would leave that LDS idle and cap the waves per CU (64-thread blocks: 2 per
+// SIMD).
 template <class PS>
 hipError_t rotations_and_strains(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated,
                                  int polar_fp64, hipStream_t s) {
     const bool k = (compensated & kKahanForces) != 0;
-    if (k && polar_fp64) return launch_modes<RotationsFor<true, true>::template M>(ps, f, c, s);
-    if (k) return launch_modes<RotationsFor<true, false>::template M>(ps, f, c, s);
-    if (polar_fp64) return launch_modes<RotationsFor<false, true>::template M>(ps, f, c, s);
-    return launch_modes<RotationsFor<false, false>::template M>(ps, f, c, s);
+    if (k && polar_fp64) return launch_modes<RotationsFor<true, true>::template M>(ps, f, c, s, kBlock);
+    if (k) return launch_modes<RotationsFor<true, false>::template M>(ps, f, c, s, kBlock);
+    if (polar_fp64) return launch_modes<RotationsFor<false, true>::template M>(ps, f, c, s, kBlock);
+    return launch_modes<RotationsFor<false, false>::template M>(ps, f, c, s, kBlock);
 }
 
 template <class PS>
@@ -568,10 +573,10 @@ template <class PS>
 hipError_t deformations_svk(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated, int polar_fp64,
                             hipStream_t s) {
     const bool k = (compensated & kKahanForces) != 0;
-    if (k && polar_fp64) return launch_modes<DeformationsFor<true, true>::template M>(ps, f, c, s);
-    if (k) return launch_modes<DeformationsFor<true, false>::template M>(ps, f, c, s);
-    if (polar_fp64) return launch_modes<DeformationsFor<false, true>::template M>(ps, f, c, s);
-    return launch_modes<DeformationsFor<false, false>::template M>(ps, f, c, s);
+    if (k && polar_fp64) return launch_modes<DeformationsFor<true, true>::template M>(ps, f, c, s, kBlock);
+    if (k) return launch_modes<DeformationsFor<true, false>::template M>(ps, f, c, s, kBlock);
+    if (polar_fp64) return launch_modes<DeformationsFor<false, true>::template M>(ps, f, c, s, kBlock);
+    return launch_modes<DeformationsFor<false, false>::template M>(ps, f, c, s, kBlock);
 }
 
 template <class PS>
@@ -585,10 +590,11 @@ template <class PS>
 hipError_t forces_solids(const PS &ps, const FieldPtrs &f, const PairCache &c, bool corotational, int compensated,
                          hipStream_t s) {
     const bool k = (compensated & kKahanForces) != 0;
-    if (corotational && k) return launch_modes<ForcesFor<true, true>::template M>(ps, f, c, s);
-    if (corotational) return launch_modes<ForcesFor<true, false>::template M>(ps, f, c, s);
-    if (k) return launch_modes<ForcesFor<false, true>::template M>(ps, f, c, s);
-    return launch_modes<ForcesFor<false, false>::template M>(ps, f, c, s);
+    const unsigned nb = neighbour_block();
+    if (corotational && k) return launch_modes<ForcesFor<true, true>::template M>(ps, f, c, s, nb);
+    if (corotational) return launch_modes<ForcesFor<true, false>::template M>(ps, f, c, s, nb);
+    if (k) return launch_modes<ForcesFor<false, true>::template M>(ps, f, c, s, nb);
+    return launch_modes<ForcesFor<false, false>::template M>(ps, f, c, s, nb);
 }
 
 }  // namespace
