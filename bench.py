@@ -278,7 +278,12 @@ def run_served(args):
         f"{sum(r.n * r.steps for r in reqs):.4g} particle-steps, last arrival {reqs[-1].arrival:.1f} s")
 
     if args.counters_only:
-        # Only the profiled pass, for hardware-counter runs under rocprofv3.
+        # Only the profiled pass, for hardware-counter runs under rocprofv3,
+        # after a batched warm-up like the one of the full run, so that the
+        # traced pass meets the device in the same state as the bench's own.
+        warm = serve.make_requests(count=max(args.warmup, 1), rate=args.rate, seed=args.seed + 1, steps=4)
+        serve.serve_batched(warm, hbm_budget=budget, realtime=False, compensated_sums=sums,
+                            graph_steps=args.graph_steps)
         prof = serve.serve_batched(reqs[:args.profile_requests], hbm_budget=budget, realtime=False,
                                    compensated_sums=sums, profile=True)
         return {"counters_only": True, "requests": args.profile_requests,

@@ -122,7 +122,9 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
     takes requests while the first is full: under a saturated queue the
     device then runs one pool's round while the host serves the other and
     does not wait for the host between rounds, and a light load runs in one
-    pool as with `pools=1`.  Profiling uses one pool.
+    pool as with `pools=1`.  Profiling uses one pool and waits for every
+    preparation it reaches, so that a profiled pass admits the same scenes
+    in the same rounds however fast the workers run.
     on_result(request, {field: array}) receives each result."""
     from .batch import Batch
 
@@ -176,7 +178,7 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
             if fut is None:
                 break
             if not fut.done():
-                if any(running) or admitted:
+                if (any(running) or admitted) and not profile:
                     break
                 tp = time.perf_counter()
                 concurrent.futures.wait([fut])       # nothing to step: wait for the scene
