@@ -89,6 +89,21 @@ def measured_traffic(workload, kernel):
     return int(sum(r["read_bytes"] + r["write_bytes"] for r in rows)) if rows else None
 
 
+def measured_issue(workload, kernel):
+    """Share of CU cycles in which pass `kernel` issued a wave64 VALU
+    instruction (rocprofv3 SQ counters, summary committed under profiles/),
+    or None.  The neighbour passes are bound by that issue rate, not by HBM."""
+    path = os.path.join(REPO, "profiles", "neighbour_passes_instruction_mix.json")
+    table = BATCH_KERNEL_SYMBOLS if workload == "served_stream" else KERNEL_SYMBOLS
+    prefix = table.get(kernel)
+    if prefix is None or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rows = [r for name, r in json.load(f).get(workload, {}).items() if name.startswith(prefix)]
+    waves = sum(r["waves"] for r in rows)
+    return round(sum(r["valu_issue"] * r["waves"] for r in rows) / waves, 3) if waves else None
+
+
 def copy_probe(mib):
     """One device-to-device copy of `mib` MiB (a copy kernel of known size):
     profiled beside the workload, its FETCH_SIZE / WRITE_SIZE readings give
@@ -201,7 +216,8 @@ def run_single(args):
         roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.MI355X_HBM_GBS,
                 "unit": "GB/s", "frac": round(achieved / roofline.MI355X_HBM_GBS, 4),
                 "traffic": measured_traffic(scene.name, dominant),
-                "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_s * 1e3, 5)}
+                "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_s * 1e3, 5),
+                "valu_issue": measured_issue(scene.name, dominant)}
     extra = {}
     if scene.system == "fluid":
         candidates, in_range = grid_pair_counts(system, scene)
@@ -328,6 +344,7 @@ def run_served(args):
         roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.MI355X_HBM_GBS,
                 "unit": "GB/s", "frac": round(achieved / roofline.MI355X_HBM_GBS, 5),
                 "traffic": measured_traffic("served_stream", dominant),
+                "valu_issue": measured_issue("served_stream", dominant),
                 "bytes_per_launch": round(bytes_launch), "avg_launch_ms": round(avg_s * 1e3, 5),
                 "sample": f"first {len(reqs[:args.profile_requests])} requests, saturated, no graphs"}
     pass_ms = {k: round(ms / max(n, 1), 5) for k, (n, ms) in sorted(stats.items(), key=lambda x: -x[1][1])}
