@@ -100,8 +100,10 @@ def measured_issue(workload, kernel):
         return None
     with open(path) as f:
         rows = [r for name, r in json.load(f).get(workload, {}).items() if name.startswith(prefix)]
-    waves = sum(r["waves"] for r in rows)
-    return round(sum(r["valu_issue"] * r["waves"] for r in rows) / waves, 3) if waves else None
+    # kernels of one pass (list and overflow launches): issued VALU over cycles
+    valu = sum(r["valu_per_wave"] * r["waves"] for r in rows)
+    cycles = sum(r["valu_per_wave"] * r["waves"] / r["valu_issue"] for r in rows if r["valu_issue"] > 0)
+    return round(valu / cycles, 3) if cycles else None
 
 
 def copy_probe(mib):
