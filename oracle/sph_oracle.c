@@ -18,7 +18,8 @@
  *   - pow(h, k) and the kernel constants: 315/(64 pi h^9), -45/(pi h^6),
  *     45/(pi h^6) evaluated in double and rounded once;
  *   - length(v) = sqrt((x*x + y*y) + z*z), IEEE sqrt, no FMA contraction;
- *   - rsqrt(x) = 1/sqrt(x); atan/sin/cos from the C library;
+ *   - rsqrt(x) = 1/sqrt(x); atan/sin/cos from the C library in the FP32
+ *     Jacobi, half-angle identities in the FP64 one (half_atan_cs_d);
  *   - a Givens rotation whose off-diagonal is exactly 0 is the identity;
  *   - cell coordinate: NaN and negative values map to 0, then clamp;
  *   - the SVK Cauchy stress keeps all nine entries.
@@ -370,14 +371,49 @@ static vec mat_mv(const real *m, vec v) {
 }
 
 /* Generic-precision helpers for the Jacobi solver (prec = float or double). */
-#define DEFINE_JACOBI(T, SUF, ATAN, SIN, COS, SQRT, ABS)                                               \
+/* cos and sin of angle = 1/2 atan(e / d) (kernels/common.cl:101-155).  FP32:
+ * the reference's library calls on the reference's quotient.  FP64 (the
+ * precision upgrade, which the reference does not have): the same angle from
+ * r = sqrt(d^2 + e^2) without library trig, cos phi = |d| / r,
+ * sin phi = sign(d) e / r, cos(phi/2) = sqrt(1/2 + cos(phi)/2),
+ * sin(phi/2) = sin phi / (2 cos(phi/2)); the quotient form when r is not
+ * finite.  Both are the engine's exact operation sequences. */
+static void half_atan_cs_f(float e, float d, float *c, float *s) {
+    float angle = 0.5f * atanf(e / d);
+    *c = cosf(angle);
+    *s = sinf(angle);
+}
+static void half_atan_cs_d(double e, double d, double *c, double *s) {
+    double r = sqrt(d * d + e * e), cphi, sphi, ch;
+    if (r <= 1.7e308) {
+        double inv = 1.0 / r;
+        cphi = fabs(d) * inv;
+        sphi = (signbit(d) ? -e : e) * inv;
+    } else {
+        double x = e / d;
+        if (fabs(x) <= 1.0) {
+            double q = 1.0 / sqrt(1.0 + x * x);
+            cphi = q;
+            sphi = x * q;
+        } else {
+            double ix = 1.0 / fabs(x);
+            double q = 1.0 / sqrt(1.0 + ix * ix);
+            cphi = ix * q;
+            sphi = copysign(q, x);
+        }
+    }
+    ch = sqrt(0.5 + 0.5 * cphi);
+    *c = ch;
+    *s = sphi / (ch + ch);
+}
+
+#define DEFINE_JACOBI(T, SUF, HALF_ATAN_CS, ABS)                                                       \
     static void givens_##SUF(const T *a, const T *p, int od, T *ao, T *po) {                           \
         static const int XX[3] = {0, 0, 1}, YY[3] = {1, 2, 2}, EE[3] = {2, 1, 0}, EX[3] = {4, 3, 3},    \
                          EY[3] = {5, 5, 4};                                                            \
         int k = od - 3, ixx = XX[k], iyy = YY[k], iee = EE[k], iex = EX[k], iey = EY[k], r;             \
-        T angle = 0;                                                                                   \
-        if (a[od] != 0) angle = (T)0.5 * ATAN((T)2 * a[od] / (a[ixx] - a[iyy]));                       \
-        T c = COS(angle), s = SIN(angle);                                                              \
+        T c = 1, s = 0;                                                                                \
+        if (a[od] != 0) HALF_ATAN_CS((T)2 * a[od], a[ixx] - a[iyy], &c, &s);                           \
         T c2 = c * c, s2 = s * s, cs = c * s;                                                          \
         ao[ixx] = a[ixx] * c2 + a[iyy] * s2 + (T)2 * a[od] * cs;                                       \
         ao[od] = -a[ixx] * cs + a[iyy] * cs + a[od] * (c2 - s2);                                       \
@@ -433,13 +469,9 @@ static vec mat_mv(const real *m, vec v) {
                                a[3 * r + 2] * s[sidx(2, c)];                                           \
     }
 
-static float atanf_(float x) { return atanf(x); }
-static float sinf_(float x) { return sinf(x); }
-static float cosf_(float x) { return cosf(x); }
-static float sqrtf_(float x) { return sqrtf(x); }
 static float fabsf_(float x) { return fabsf(x); }
-DEFINE_JACOBI(float, f, atanf_, sinf_, cosf_, sqrtf_, fabsf_)
-DEFINE_JACOBI(double, d, atan, sin, cos, sqrt, fabs)
+DEFINE_JACOBI(float, f, half_atan_cs_f, fabsf_)
+DEFINE_JACOBI(double, d, half_atan_cs_d, fabs)
 
 /* R = A (A^T A)^(-1/2)   (getR, kernels/common.cl:306-331) */
 static void polar(const real *apq, real *rot, int fp64) {
