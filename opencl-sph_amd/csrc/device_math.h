@@ -189,8 +189,55 @@ __device__ __forceinline__ double t_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float t_abs(float x) { return fabsf(x); }
 __device__ __forceinline__ double t_abs(double x) { return fabs(x); }
 
-// cos and sin of angle = 1/2 atan(e / d).  FP32: the reference's library
-// calls on the reference's quotient.  FP64 (the precision upgrade, not bound
+// The FP32 path's atan, sin and cos.  The reference calls its OpenCL
+// platform's library (under -cl-fast-relaxed-math); here they are defined
+// once, as double polynomials rounded to float, and the oracle writes out the
+// same operation sequence, so the two agree bit for bit (within a float ulp
+// of the C library):
+//   atan: |x| > 1 -> pi/2 - atan(1/|x|); then t > tan(pi/12) ->
+//         pi/6 + atan((t sqrt3 - 1) / (t + sqrt3)); then the Taylor series
+//         t sum_{n<=15} (-t^2)^n / (2n+1) (|t| <= 0.268); odd in x;
+//   sin, cos on |a| <= pi/4 + 1e-6 (the Jacobi half angle): Taylor series
+//         to a^19 and a^20.
+__device__ __forceinline__ double poly_atan(double x) {
+    double t = fabs(x);
+    const bool inv = t > 1.0;
+    if (inv) t = 1.0 / t;
+    const bool shift = t > 0.2679491924311227;
+    if (shift) t = (t * 1.7320508075688772 - 1.0) / (t + 1.7320508075688772);
+    const double u = t * t;
+    double p = 1.0 / 31.0;
+    p = -p;
+#pragma unroll
+    for (int n = 14; n >= 0; --n) p = p * u + ((n & 1) ? -1.0 : 1.0) / static_cast<double>(2 * n + 1);
+    double r = t * p;
+    if (shift) r = 0.5235987755982989 + r;
+    if (inv) r = 1.5707963267948966 - r;
+    return copysign(r, x);
+}
+__device__ __forceinline__ double poly_sin(double a) {
+    const double c[10] = {1.0, -1.0 / 6.0, 1.0 / 120.0, -1.0 / 5040.0, 1.0 / 362880.0,
+                          -1.0 / 39916800.0, 1.0 / 6227020800.0, -1.0 / 1307674368000.0,
+                          1.0 / 355687428096000.0, -1.0 / 121645100408832000.0};
+    const double u = a * a;
+    double p = c[9];
+#pragma unroll
+    for (int n = 8; n >= 0; --n) p = p * u + c[n];
+    return a * p;
+}
+__device__ __forceinline__ double poly_cos(double a) {
+    const double c[11] = {1.0, -1.0 / 2.0, 1.0 / 24.0, -1.0 / 720.0, 1.0 / 40320.0,
+                          -1.0 / 3628800.0, 1.0 / 479001600.0, -1.0 / 87178291200.0,
+                          1.0 / 20922789888000.0, -1.0 / 6402373705728000.0,
+                          1.0 / 2432902008176640000.0};
+    const double u = a * a;
+    double p = c[10];
+#pragma unroll
+    for (int n = 9; n >= 0; --n) p = p * u + c[n];
+    return p;
+}
+// cos and sin of angle = 1/2 atan(e / d).  FP32: the reference's atan, cos and
+// sin (poly_*) on the reference's quotient.  FP64 (the precision upgrade, not bound
 // to the reference's bits): the same angle without library trig, from
 // r = sqrt(d^2 + e^2): cos phi = |d| / r, sin phi = sign(d) e / r (phi in
 // (-pi/2, pi/2), the range of atan; sign(-0) = -1 as e / -0 = -inf has it),
@@ -199,9 +246,9 @@ __device__ __forceinline__ double t_abs(double x) { return fabs(x); }
 // d^2 + e^2 cannot overflow; a non-finite r (infinite or NaN input) takes
 // the quotient form.  Agrees with the library calls to a few FP64 ulps.
 __device__ __forceinline__ void half_atan_cs(float e, float d, float *c, float *s) {
-    const float angle = 0.5f * atanf(e / d);
-    *c = cosf(angle);
-    *s = sinf(angle);
+    const float angle = 0.5f * static_cast<float>(poly_atan(static_cast<double>(e / d)));
+    *c = static_cast<float>(poly_cos(static_cast<double>(angle)));
+    *s = static_cast<float>(poly_sin(static_cast<double>(angle)));
 }
 __device__ __forceinline__ void half_atan_cs(double e, double d, double *c, double *s) {
     const double r = sqrt(d * d + e * e);

@@ -18,8 +18,9 @@
  *   - pow(h, k) and the kernel constants: 315/(64 pi h^9), -45/(pi h^6),
  *     45/(pi h^6) evaluated in double and rounded once;
  *   - length(v) = sqrt((x*x + y*y) + z*z), IEEE sqrt, no FMA contraction;
- *   - rsqrt(x) = 1/sqrt(x); atan/sin/cos from the C library in the FP32
- *     Jacobi, half-angle identities in the FP64 one (half_atan_cs_d);
+ *   - rsqrt(x) = 1/sqrt(x); in the FP32 Jacobi, atan/sin/cos are double
+ *     polynomials rounded to float (orc_atan_d, orc_sin_d, orc_cos_d), in
+ *     the FP64 one half-angle identities (half_atan_cs_d);
  *   - a Givens rotation whose off-diagonal is exactly 0 is the identity;
  *   - cell coordinate: NaN and negative values map to 0, then clamp;
  *   - the SVK Cauchy stress keeps all nine entries.
@@ -378,10 +379,56 @@ static vec mat_mv(const real *m, vec v) {
  * sin phi = sign(d) e / r, cos(phi/2) = sqrt(1/2 + cos(phi)/2),
  * sin(phi/2) = sin phi / (2 cos(phi/2)); the quotient form when r is not
  * finite.  Both are the engine's exact operation sequences. */
+/* The FP32 path's atan, sin and cos.  The reference calls the OpenCL
+ * platform's library under -cl-fast-relaxed-math, whose results are the
+ * platform's own; the engine and this oracle define them once, as double
+ * polynomials rounded to float, written out identically on both sides so
+ * that their bits agree (pinned against the C library to a float ulp):
+ *   atan: |x| > 1 -> pi/2 - atan(1/|x|); then t > tan(pi/12) ->
+ *         pi/6 + atan((t sqrt3 - 1) / (t + sqrt3)); then the Taylor series
+ *         t sum_{n<=15} (-t^2)^n / (2n+1) (|t| <= 0.268); odd in x;
+ *   sin, cos on |a| <= pi/4 + 1e-6 (the Jacobi half angle): Taylor series
+ *         to a^19 and a^20. */
+double orc_atan_d(double x) {
+    double t = fabs(x), u, p, r;
+    int inv = t > 1.0, shift = 0, n;
+    if (inv) t = 1.0 / t;
+    if (t > 0.2679491924311227) {
+        t = (t * 1.7320508075688772 - 1.0) / (t + 1.7320508075688772);
+        shift = 1;
+    }
+    u = t * t;
+    p = 1.0 / 31.0;
+    p = -p;
+    for (n = 14; n >= 0; --n) p = p * u + ((n & 1) ? -1.0 : 1.0) / (double)(2 * n + 1);
+    r = t * p;
+    if (shift) r = 0.5235987755982989 + r;
+    if (inv) r = 1.5707963267948966 - r;
+    return copysign(r, x);
+}
+double orc_sin_d(double a) {
+    static const double c[10] = {1.0, -1.0 / 6.0, 1.0 / 120.0, -1.0 / 5040.0, 1.0 / 362880.0,
+                                 -1.0 / 39916800.0, 1.0 / 6227020800.0, -1.0 / 1307674368000.0,
+                                 1.0 / 355687428096000.0, -1.0 / 121645100408832000.0};
+    double u = a * a, p = c[9];
+    int n;
+    for (n = 8; n >= 0; --n) p = p * u + c[n];
+    return a * p;
+}
+double orc_cos_d(double a) {
+    static const double c[11] = {1.0, -1.0 / 2.0, 1.0 / 24.0, -1.0 / 720.0, 1.0 / 40320.0,
+                                 -1.0 / 3628800.0, 1.0 / 479001600.0, -1.0 / 87178291200.0,
+                                 1.0 / 20922789888000.0, -1.0 / 6402373705728000.0,
+                                 1.0 / 2432902008176640000.0};
+    double u = a * a, p = c[10];
+    int n;
+    for (n = 9; n >= 0; --n) p = p * u + c[n];
+    return p;
+}
 static void half_atan_cs_f(float e, float d, float *c, float *s) {
-    float angle = 0.5f * atanf(e / d);
-    *c = cosf(angle);
-    *s = sinf(angle);
+    float angle = 0.5f * (float)orc_atan_d((double)(e / d));
+    *c = (float)orc_cos_d((double)angle);
+    *s = (float)orc_sin_d((double)angle);
 }
 static void half_atan_cs_d(double e, double d, double *c, double *s) {
     double r = sqrt(d * d + e * e), cphi, sphi, ch;
