@@ -33,6 +33,7 @@ def _params_type(real):
             ("nconstraints", c_uint), ("constraints", (real * 6) * MAXC),
             ("body_count", c_uint), ("polar_fp64", c_int),
             ("eos_tait", c_int), ("mass_consistent", c_int), ("svk_szz_zero", c_int),
+            ("compensated", c_int),
         ]
     return Params
 
@@ -98,6 +99,7 @@ class OracleState:
         p.eos_tait = int(params.get("eos_tait", 0))
         p.mass_consistent = int(params.get("mass_consistent", 0))
         p.svk_szz_zero = int(params.get("svk_sigma_zz_zero", 0))
+        p.compensated = int(params.get("compensated", 0))
         self.params = p
         self.ncells = res[0] * res[1] * res[2]
         n = self.n

@@ -56,7 +56,16 @@ typedef struct {
     int mass_consistent;  /* 1: fluid force times the particle's own mass */
     int svk_szz_zero;     /* 1: SVK sigma_zz stored as 0, what the reference's out-of-bounds read of a
                              float[6] gives on its platform (kernels/solid_svk.cl:48,75) */
+    int compensated;      /* bit 0: Kahan-compensated density sums (the engine's precision option) */
 } orc_params;
+
+/* s += v with Kahan compensation c, the engine's operation sequence */
+static void kahan_add(real *s, real *c, real v) {
+    real y = v - *c;
+    real t = *s + y;
+    *c = (t - *s) - y;
+    *s = t;
+}
 
 typedef struct {
     real h, h2, poly6, spiky, lap;
@@ -215,7 +224,8 @@ static void density_impl(const orc_params *p, const real *pos, const unsigned *g
         unsigned cells[27];
         int nc = cells_around(p, gridcell[i], cells), c;
         vec xi = ld(pos, (unsigned)i);
-        real rho = 0;
+        real rho = 0, comp = 0;
+        const int kahan = p->compensated & 1;
         unsigned nn = 0;
         for (c = 0; c < nc; ++c) {
             unsigned jp, off = celloffset[cells[c]];
@@ -223,13 +233,17 @@ static void density_impl(const orc_params *p, const real *pos, const unsigned *g
                 unsigned j = cellparticles[jp], b;
                 vec xj = ld(pos, j);
                 real r = vlen(vsub(xj, xi));
-                if (!(r > k.h)) rho = rho + W(r, &k);
+                if (!(r > k.h)) {
+                    if (kahan) kahan_add(&rho, &comp, W(r, &k));
+                    else rho = rho + W(r, &k);
+                }
                 if (r < k.h) ++nn;
                 for (b = 1; b < ncombo; ++b) {
                     if (!combo_ok(p, b, xi, xj)) continue;
                     real rr = vlen(vsub(reflect_combo(p, b, xj), xi));
                     if (rr < k.h) {
-                        rho = rho + W(rr, &k);
+                        if (kahan) kahan_add(&rho, &comp, W(rr, &k));
+                        else rho = rho + W(rr, &k);
                         ++nn;
                     }
                 }

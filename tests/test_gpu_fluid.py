@@ -3,11 +3,10 @@ integrator after 1 and 100 steps, both with the reference's own model
 (linear EOS, force without the particle's mass) and with the stable model
 the benchmarks use (Tait EOS, mass-consistent force).
 
-Tolerance: the engine performs the same IEEE operations in the same order
-as the oracle (no FMA contraction, correctly rounded division and sqrt), so
-the expected difference is zero; the asserted bound is rtol 1e-5 of the
-value plus atol 1e-6 of the field's scale, which any reordering of the sums
-would exceed by orders of magnitude on a diverging run."""
+Bitwise: the engine performs the same IEEE operations in the same order as
+the oracle (no FMA contraction, correctly rounded division and sqrt), with
+plain and with Kahan-compensated density sums, so every field must match
+exactly."""
 import numpy as np
 import pytest
 
@@ -19,9 +18,9 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("density", "num_neighbours", "force", "acceleration", "velocity", "position")
 
 
-def run_pair(scene, steps):
-    s = engine(scene)
-    st = oracle_state(scene)
+def run_pair(scene, steps, compensated=0):
+    s = engine(scene, compensated=compensated)
+    st = oracle_state(scene, params={"compensated": compensated})
     s.run(steps)
     st.step("fluid", steps)
     got = download(s, *FIELDS)
@@ -30,9 +29,9 @@ def run_pair(scene, steps):
 
 
 def check(got, st, exact_counts=True):
-    scale = {k: float(np.nanmax(np.abs(st.f[k]))) if k in st.f else 1.0 for k in FIELDS}
+    """Bitwise: the engine runs the oracle's operation sequence."""
     for k in ("density", "force", "acceleration", "velocity", "position"):
-        compare(k, got[k], st.f[k], rtol=1e-5, atol=1e-6 * scale[k])
+        compare(k, got[k], st.f[k])
     if exact_counts:
         assert np.array_equal(got["num_neighbours"], st.num_neighbours)
 
@@ -112,3 +111,13 @@ def test_large_dam_break_properties(hip_device):
     rho = got["density"]
     assert np.all(np.isfinite(rho)) and rho.min() > 0
     s.free()
+
+
+@pytest.mark.parametrize("steps", [1, 20])
+def test_kahan_density_bitwise(hip_device, steps):
+    """Kahan-compensated density sums (the benchmarks' setting) against the
+    oracle's compensated sums: the same operation sequence, bit for bit."""
+    scene = scenes.dam_break(24, 24, 12, 0.02, 0.04, 1e-4, seed=7, eos_tait=True, mass_consistent=True)
+    say(f"fluid {steps} step(s), Kahan density, n={scene.n}")
+    got, st = run_pair(scene, steps, compensated=1)
+    check(got, st)

@@ -60,10 +60,10 @@ def small_solid(kind, nside=9, stretch=1.01, planes=True, constraints=True, bodi
     return scene, strained, vel
 
 
-def run_both(kind, steps, polar_fp64=True, **kw):
+def run_both(kind, steps, polar_fp64=True, compensated=0, **kw):
     scene, strained, vel = small_solid(kind, **kw)
-    s = engine(scene, polar_fp64=polar_fp64)
-    st = oracle_state(scene, polar_fp64=polar_fp64)
+    s = engine(scene, polar_fp64=polar_fp64, compensated=compensated)
+    st = oracle_state(scene, polar_fp64=polar_fp64, params={"compensated": compensated})
     # reference configuration from the unstrained lattice, then deform
     got0 = download(s, "originalpos", "density0")
     compare("density0", got0["density0"], st.f["density0"])
@@ -109,5 +109,15 @@ def test_solid_passes_bitwise(hip_device, kind, steps, polar_fp64):
     a static layer, and 20 steps without the planes."""
     say(f"{kind}: {steps} step(s), {'FP64' if polar_fp64 else 'FP32'} polar, bitwise")
     got, st = run_both(kind, steps, polar_fp64=polar_fp64, planes=steps == 1)
+    for k, v in got.items():
+        compare(k, v, st.f[k].reshape(v.shape))
+
+
+@pytest.mark.parametrize("kind", ["corotational", "svk"])
+def test_kahan_density_bitwise(hip_device, kind):
+    """Kahan-compensated density and rest-density sums (the benchmarks'
+    setting) against the oracle's compensated sums, bit for bit."""
+    say(f"{kind}: 5 steps, Kahan density, bitwise")
+    got, st = run_both(kind, 5, compensated=1)
     for k, v in got.items():
         compare(k, v, st.f[k].reshape(v.shape))
