@@ -56,7 +56,8 @@ typedef struct {
     int mass_consistent;  /* 1: fluid force times the particle's own mass */
     int svk_szz_zero;     /* 1: SVK sigma_zz stored as 0, what the reference's out-of-bounds read of a
                              float[6] gives on its platform (kernels/solid_svk.cl:48,75) */
-    int compensated;      /* bit 0: Kahan-compensated density sums (the engine's precision option) */
+    int compensated;      /* Kahan-compensated sums (the engine's precision option): bit 0 density and
+                             rest density, bit 1 forces, A_pq / A_qq and the displacement gradient */
 } orc_params;
 
 /* s += v with Kahan compensation c, the engine's operation sequence */
@@ -65,6 +66,11 @@ static void kahan_add(real *s, real *c, real v) {
     real t = *s + y;
     *c = (t - *s) - y;
     *s = t;
+}
+/* s += v, compensated when kahan is set */
+static void acc_add(real *s, real *c, real v, int kahan) {
+    if (kahan) kahan_add(s, c, v);
+    else *s = *s + v;
 }
 
 typedef struct {
@@ -296,7 +302,8 @@ void orc_forces_fluids(const orc_params *p, const real *pos, const real *vel, co
         int nc = cells_around(p, gridcell[i], cells), c;
         vec xi = ld(pos, (unsigned)i), vi = ld(vel, (unsigned)i);
         real pi = pressure(p, rho[i]);
-        vec f = {0, 0, 0};
+        vec f = {0, 0, 0}, fc = {0, 0, 0};
+        const int kahan = (p->compensated & 2) != 0;
         for (c = 0; c < nc; ++c) {
             unsigned jp, off = celloffset[cells[c]];
             for (jp = off; jp < off + gridcount[cells[c]]; ++jp) {
@@ -312,9 +319,9 @@ void orc_forces_fluids(const orc_params *p, const real *pos, const real *vel, co
                 real didj = rho[i] * rho[j];
                 real vterm = (p->viscosity * din) * k.lap;
                 real pterm = (real)((double)(pi + pj) * 0.5 / (double)r * (double)din * (double)din * (double)k.spiky);
-                f.x = f.x + (d.x * pterm + rv.x * vterm) / didj;
-                f.y = f.y + (d.y * pterm + rv.y * vterm) / didj;
-                f.z = f.z + (d.z * pterm + rv.z * vterm) / didj;
+                acc_add(&f.x, &fc.x, (d.x * pterm + rv.x * vterm) / didj, kahan);
+                acc_add(&f.y, &fc.y, (d.y * pterm + rv.y * vterm) / didj, kahan);
+                acc_add(&f.z, &fc.z, (d.z * pterm + rv.z * vterm) / didj, kahan);
             }
         }
         f = vmuls(f, p->mass);
@@ -597,22 +604,33 @@ void orc_polar(const real *apq, real *rot, int fp64) { polar(apq, rot, fp64); }
 void orc_jacobi_inverse_apply(const real *apq, const real *aqq, real *def, int fp64) { deformation(apq, aqq, def, fp64); }
 
 /* --------------------------------------------------- corotational solid */
-static void add_apq(real *apq, vec pv, vec q, real w) {
+/* The sums below take a compensation array c of the same shape (used when
+ * kahan is set). */
+static void add_apq(real *apq, real *c, vec pv, vec q, real w, int kahan) {
     vec wp = vscale(w, pv);
-    apq[0] = apq[0] + wp.x * q.x; apq[1] = apq[1] + wp.x * q.y; apq[2] = apq[2] + wp.x * q.z;
-    apq[3] = apq[3] + wp.y * q.x; apq[4] = apq[4] + wp.y * q.y; apq[5] = apq[5] + wp.y * q.z;
-    apq[6] = apq[6] + wp.z * q.x; apq[7] = apq[7] + wp.z * q.y; apq[8] = apq[8] + wp.z * q.z;
+    acc_add(&apq[0], &c[0], wp.x * q.x, kahan); acc_add(&apq[1], &c[1], wp.x * q.y, kahan);
+    acc_add(&apq[2], &c[2], wp.x * q.z, kahan); acc_add(&apq[3], &c[3], wp.y * q.x, kahan);
+    acc_add(&apq[4], &c[4], wp.y * q.y, kahan); acc_add(&apq[5], &c[5], wp.y * q.z, kahan);
+    acc_add(&apq[6], &c[6], wp.z * q.x, kahan); acc_add(&apq[7], &c[7], wp.z * q.y, kahan);
+    acc_add(&apq[8], &c[8], wp.z * q.z, kahan);
 }
-static void add_aqq(real *aqq, vec q, real w) {
+static void add_aqq(real *aqq, real *c, vec q, real w, int kahan) {
     vec wq = vscale(w, q);
-    aqq[0] = aqq[0] + wq.x * q.x; aqq[3] = aqq[3] + wq.x * q.y; aqq[4] = aqq[4] + wq.x * q.z;
-    aqq[1] = aqq[1] + wq.y * q.y; aqq[5] = aqq[5] + wq.y * q.z;
-    aqq[2] = aqq[2] + wq.z * q.z;
+    acc_add(&aqq[0], &c[0], wq.x * q.x, kahan); acc_add(&aqq[3], &c[3], wq.x * q.y, kahan);
+    acc_add(&aqq[4], &c[4], wq.x * q.z, kahan); acc_add(&aqq[1], &c[1], wq.y * q.y, kahan);
+    acc_add(&aqq[5], &c[5], wq.y * q.z, kahan); acc_add(&aqq[2], &c[2], wq.z * q.z, kahan);
 }
-static void add_outer(real *o, vec a, vec b) {
-    o[0] = o[0] + a.x * b.x; o[1] = o[1] + a.x * b.y; o[2] = o[2] + a.x * b.z;
-    o[3] = o[3] + a.y * b.x; o[4] = o[4] + a.y * b.y; o[5] = o[5] + a.y * b.z;
-    o[6] = o[6] + a.z * b.x; o[7] = o[7] + a.z * b.y; o[8] = o[8] + a.z * b.z;
+static void add_outer(real *o, real *c, vec a, vec b, int kahan) {
+    acc_add(&o[0], &c[0], a.x * b.x, kahan); acc_add(&o[1], &c[1], a.x * b.y, kahan);
+    acc_add(&o[2], &c[2], a.x * b.z, kahan); acc_add(&o[3], &c[3], a.y * b.x, kahan);
+    acc_add(&o[4], &c[4], a.y * b.y, kahan); acc_add(&o[5], &c[5], a.y * b.z, kahan);
+    acc_add(&o[6], &c[6], a.z * b.x, kahan); acc_add(&o[7], &c[7], a.z * b.y, kahan);
+    acc_add(&o[8], &c[8], a.z * b.z, kahan);
+}
+static void vacc_add(vec *s, vec *c, vec v, int kahan) {
+    acc_add(&s->x, &c->x, v.x, kahan);
+    acc_add(&s->y, &c->y, v.y, kahan);
+    acc_add(&s->z, &c->z, v.z, kahan);
 }
 
 /* compute_rotations_and_strains (kernels/solid_corotation.cl:28-195) */
@@ -629,7 +647,8 @@ void orc_rotations_strains(const orc_params *p, const real *pos, const real *opo
         unsigned cells[27];
         int nc = cells_around(p, gridcell[i], cells), c, t;
         vec xi = ld(pos, (unsigned)i), x0i = ld(opos, (unsigned)i);
-        real apq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        real apq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, capq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int kahan = (p->compensated & 2) != 0;
         for (c = 0; c < nc; ++c) {
             unsigned jp, off = celloffset[cells[c]];
             for (jp = off; jp < off + gridcount[cells[c]]; ++jp) {
@@ -638,14 +657,14 @@ void orc_rotations_strains(const orc_params *p, const real *pos, const real *opo
                 if (j != (unsigned)i) {
                     vec pv = vsub(xj, xi);
                     real r = vlen(pv);
-                    if (!(r > k.h)) add_apq(apq, pv, vsub(x0j, x0i), W(r, &k));
+                    if (!(r > k.h)) add_apq(apq, capq, pv, vsub(x0j, x0i), W(r, &k), kahan);
                 }
                 for (b = 1; b < ncombo; ++b) {
                     if (!combo_ok(p, b, xi, xj)) continue;
                     vec pv = vsub(reflect_combo(p, b, xj), xi);
                     vec q = vsub(reflect_combo(p, b, x0j), x0i);
                     real d = vlen(pv);
-                    if (d < k.h) add_apq(apq, pv, q, W(d, &k));
+                    if (d < k.h) add_apq(apq, capq, pv, q, W(d, &k), kahan);
                 }
             }
         }
@@ -653,7 +672,7 @@ void orc_rotations_strains(const orc_params *p, const real *pos, const real *opo
         real *rot = rotation + 9 * i;
         polar(apq, rot, p->polar_fp64);
         real rt[9] = {rot[0], rot[3], rot[6], rot[1], rot[4], rot[7], rot[2], rot[5], rot[8]};
-        real du[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        real du[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, cdu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         real d0i = density0[i];
         for (c = 0; c < nc; ++c) {
             unsigned jp, off = celloffset[cells[c]];
@@ -664,7 +683,7 @@ void orc_rotations_strains(const orc_params *p, const real *pos, const real *opo
                     vec q = vsub(x0j, x0i);
                     if (!(vlen(q) > k.h)) {
                         vec u = vsub(mat_mv(rt, vsub(xj, xi)), q);
-                        add_outer(du, vdivs(vmuls(u, p->mass), d0i), gradW(vneg(q), &k));
+                        add_outer(du, cdu, vdivs(vmuls(u, p->mass), d0i), gradW(vneg(q), &k), kahan);
                     }
                 }
                 for (b = 1; b < ncombo; ++b) {
@@ -672,7 +691,7 @@ void orc_rotations_strains(const orc_params *p, const real *pos, const real *opo
                     vec pv = mat_mv(rt, vsub(reflect_combo(p, b, xj), xi));
                     vec q = vsub(reflect_combo(p, b, x0j), x0i);
                     vec u = vsub(pv, q);
-                    if (vlen(pv) < k.h) add_outer(du, vdivs(vmuls(u, p->mass), d0i), gradW(vneg(q), &k));
+                    if (vlen(pv) < k.h) add_outer(du, cdu, vdivs(vmuls(u, p->mass), d0i), gradW(vneg(q), &k), kahan);
                 }
             }
         }
@@ -718,6 +737,8 @@ void orc_deformations_svk(const orc_params *p, const real *pos, const real *opos
         int nc = cells_around(p, gridcell[i], cells), c;
         vec xi = ld(pos, (unsigned)i), x0i = ld(opos, (unsigned)i);
         real apq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, aqq[6] = {0, 0, 0, 0, 0, 0};
+        real capq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, caqq[6] = {0, 0, 0, 0, 0, 0};
+        const int kahan = (p->compensated & 2) != 0;
         for (c = 0; c < nc; ++c) {
             unsigned jp, off = celloffset[cells[c]];
             for (jp = off; jp < off + gridcount[cells[c]]; ++jp) {
@@ -729,8 +750,8 @@ void orc_deformations_svk(const orc_params *p, const real *pos, const real *opos
                     if (!(r > k.h)) {
                         vec q = vsub(x0j, x0i);
                         real w = W(r, &k);
-                        add_apq(apq, pv, q, w);
-                        add_aqq(aqq, q, w);
+                        add_apq(apq, capq, pv, q, w, kahan);
+                        add_aqq(aqq, caqq, q, w, kahan);
                     }
                 }
                 for (b = 1; b < ncombo; ++b) {
@@ -740,8 +761,8 @@ void orc_deformations_svk(const orc_params *p, const real *pos, const real *opos
                     real d = vlen(pv);
                     if (d < k.h) {
                         real w = W(d, &k);
-                        add_apq(apq, pv, q, w);
-                        add_aqq(aqq, q, w);
+                        add_apq(apq, capq, pv, q, w, kahan);
+                        add_aqq(aqq, caqq, q, w, kahan);
                     }
                 }
             }
@@ -814,6 +835,8 @@ void orc_forces_solids(const orc_params *p, const real *pos, const real *vel, co
         const real *si = stress + 6 * i;
         const real *ri = rotation ? rotation + 9 * i : NULL;
         vec fe = {0, 0, 0}, fv = {0, 0, 0}, fec = {0, 0, 0}, fvc = {0, 0, 0};
+        vec cfe = {0, 0, 0}, cfv = {0, 0, 0}, cfec = {0, 0, 0}, cfvc = {0, 0, 0};
+        const int kahan = (p->compensated & 2) != 0;
         for (c = 0; c < nc; ++c) {
             unsigned jp, off = celloffset[cells[c]];
             for (jp = off; jp < off + gridcount[cells[c]]; ++jp) {
@@ -832,10 +855,10 @@ void orc_forces_solids(const orc_params *p, const real *pos, const real *vel, co
                                                                   : vscale(-vivj, sym_mv(sj, gradW(vneg(X), &k)));
                     vec e = rotation ? vadd(vneg(mat_mv(ri, fji)), mat_mv(rj, fij)) : vadd(vneg(fji), fij);
                     vec v = vmuls(vscale(mrho, vsub(vj, vi)), lapW(rcur, &k));
-                    fe = vadd(fe, e);
-                    fv = vadd(fv, v);
-                    fec = vadd(fec, e);
-                    fvc = vadd(fvc, v);
+                    vacc_add(&fe, &cfe, e, kahan);
+                    vacc_add(&fv, &cfv, v, kahan);
+                    vacc_add(&fec, &cfec, e, kahan);
+                    vacc_add(&fvc, &cfvc, v, kahan);
                 }
                 for (b = 1; b < ncombo; ++b) {
                     if (!combo_ok(p, b, xi, xj)) continue;
@@ -851,13 +874,13 @@ void orc_forces_solids(const orc_params *p, const real *pos, const real *vel, co
                             lowest = s - 1;
                         }
                     vec e = rotation ? vadd(vneg(mat_mv(ri, fji)), fij) : vadd(vneg(fji), fij);
-                    fe = vadd(fe, e);
+                    vacc_add(&fe, &cfe, e, kahan);
                     vec refvel = reflect_dir(vj, plane_nm(p, lowest));
                     vec v = vmuls(vscale(mrho, vsub(refvel, vi)), lapW(vlen(vsub(rp, xi)), &k));
-                    fv = vadd(fv, v);
+                    vacc_add(&fv, &cfv, v, kahan);
                     if (!(b & 1u)) {
-                        fec = vadd(fec, e);
-                        fvc = vadd(fvc, v);
+                        vacc_add(&fec, &cfec, e, kahan);
+                        vacc_add(&fvc, &cfvc, v, kahan);
                     }
                 }
             }

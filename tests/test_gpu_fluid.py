@@ -5,8 +5,7 @@ the benchmarks use (Tait EOS, mass-consistent force).
 
 Bitwise: the engine performs the same IEEE operations in the same order as
 the oracle (no FMA contraction, correctly rounded division and sqrt), with
-plain and with Kahan-compensated density sums, so every field must match
-exactly."""
+plain and with Kahan-compensated sums, so every field must match exactly."""
 import numpy as np
 import pytest
 
@@ -114,10 +113,12 @@ def test_large_dam_break_properties(hip_device):
 
 
 @pytest.mark.parametrize("steps", [1, 20])
-def test_kahan_density_bitwise(hip_device, steps):
-    """Kahan-compensated density sums (the benchmarks' setting) against the
-    oracle's compensated sums: the same operation sequence, bit for bit."""
+@pytest.mark.parametrize("compensated", [1, 3], ids=["kahan_density", "kahan_all"])
+def test_kahan_sums_bitwise(hip_device, steps, compensated):
+    """Kahan-compensated density sums (the benchmarks' setting), and density
+    plus force sums, against the oracle's compensated sums: the same
+    operation sequence, bit for bit."""
     scene = scenes.dam_break(24, 24, 12, 0.02, 0.04, 1e-4, seed=7, eos_tait=True, mass_consistent=True)
-    say(f"fluid {steps} step(s), Kahan density, n={scene.n}")
-    got, st = run_pair(scene, steps, compensated=1)
+    say(f"fluid {steps} step(s), compensated={compensated}, n={scene.n}")
+    got, st = run_pair(scene, steps, compensated=compensated)
     check(got, st)

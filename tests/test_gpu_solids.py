@@ -114,10 +114,12 @@ def test_solid_passes_bitwise(hip_device, kind, steps, polar_fp64):
 
 
 @pytest.mark.parametrize("kind", ["corotational", "svk"])
-def test_kahan_density_bitwise(hip_device, kind):
+@pytest.mark.parametrize("compensated", [1, 3], ids=["kahan_density", "kahan_all"])
+def test_kahan_sums_bitwise(hip_device, kind, compensated):
     """Kahan-compensated density and rest-density sums (the benchmarks'
-    setting) against the oracle's compensated sums, bit for bit."""
-    say(f"{kind}: 5 steps, Kahan density, bitwise")
-    got, st = run_both(kind, 5, compensated=1)
+    setting), and those plus the force, A_pq / A_qq and displacement-gradient
+    sums, against the oracle's compensated sums, bit for bit."""
+    say(f"{kind}: 5 steps, compensated={compensated}, bitwise")
+    got, st = run_both(kind, 5, compensated=compensated)
     for k, v in got.items():
         compare(k, v, st.f[k].reshape(v.shape))
