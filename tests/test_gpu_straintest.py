@@ -1,8 +1,8 @@
 """The strain-test harness (sph.straintest, counterpart of
 scripts/straintest.py) on the engine and, through the same harness code, on
 the CPU oracle: histories and outputs agree bit for bit, constrained faces
-follow the planes, and
-the history/output memmaps are written in the reference's layout."""
+follow the planes, and the history/output memmaps are written in the
+reference's layout."""
 import os
 
 import numpy as np
