@@ -208,7 +208,9 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
                 for (unsigned u = 0; u < kScanUnroll; ++u) {
                     const float s = sqlength(xs[u] - xi);
                     if ((u < left) & !(s >= r2_out)) {
-                        if (cnt < kListCap) col[cnt * kListBlock] = static_cast<unsigned short>(ent + u);
+                        // past the cap the last entry is overwritten: the
+                        // list is then discarded (cnt > kListCap) anyway
+                        col[min(cnt, kListCap - 1u) * kListBlock] = static_cast<unsigned short>(ent + u);
                         ++cnt;
                     }
                 }
@@ -318,9 +320,9 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                         const bool cur = !(sqlength(xs[u] - xi) >= r2_out);
                         const bool ref = !(sqlength(x0s[u] - x0i) >= r2_out);
                         if ((u < left) & (cur | ref)) {
-                            if (cnt < kListCap)
-                                col[cnt * kListBlock] = static_cast<unsigned short>(
-                                    (ent + u) | (cur ? 1u << 11 : 0u) | (ref ? 1u << 10 : 0u));
+                            // past the cap the last entry is overwritten (list discarded)
+                            col[min(cnt, kListCap - 1u) * kListBlock] = static_cast<unsigned short>(
+                                (ent + u) | (cur ? 1u << 11 : 0u) | (ref ? 1u << 10 : 0u));
                             ++cnt;
                         }
                     }
