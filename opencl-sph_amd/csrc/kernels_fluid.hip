@@ -56,6 +56,8 @@ __device__ __forceinline__ float pressure(const StepParams &p, float rho) {
 // images (kernels/common.cl:634-687).  Returns the kernel sum; counts r < h
 // in *nn (COUNT); when `list` is given, records the slots with r <= h in
 // list[k * stride + i] (at most kListCap) and their number in *listed.
+// The direct term's range test is the square-root-free form of r <= h
+// (s < r2_out), so out-of-range candidates take no square root.
 template <bool KAHAN, bool COUNT>
 __device__ __forceinline__ float density_scan(const StepParams &p, const FieldPtrs &f, unsigned i, V3 xi,
                                               unsigned inear, unsigned *nn, unsigned *list, unsigned stride,
@@ -63,17 +65,19 @@ __device__ __forceinline__ float density_scan(const StepParams &p, const FieldPt
     const KernelConsts kc = kconsts(p);
     const float *__restrict__ pos = f.position;
     const unsigned ncombo = 1u << p.nplanes;
+    const float r2_out = p.r2_out;   // read once: with a scene table p is in global memory
     Acc<KAHAN> rho;
     unsigned cnt = 0, k = 0;
     for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         const V3 xj = load3(pos, j);
-        const float r = length(xj - xi);
-        if (!(r > kc.h)) {
+        const float s = sqlength(xj - xi);
+        if (!(s >= r2_out)) {
+            const float r = sqrtf(s);
             rho.add(w_poly6(r, kc));
             if (list && k < kListCap) list[static_cast<size_t>(k) * stride + i] = j;
             ++k;
+            if (COUNT && r < kc.h) ++cnt;
         }
-        if (COUNT && r < kc.h) ++cnt;
         if (inear) {
             const unsigned ok = planes_near_j(p, xj, inear);
             if (ok)

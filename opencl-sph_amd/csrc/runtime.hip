@@ -780,9 +780,11 @@ int params_from_blob(const psdata &meta, const char *blob, const KernelSet &ks, 
     p.spiky = static_cast<float>(-45.0 / (pi * h6));
     p.lap = static_cast<float>(45.0 / (pi * h6));
     // Smallest float s with sqrtf(s) > h (sqrtf is correctly rounded on the
-    // host and on the device, and monotonic).
+    // host and on the device, and monotonic): 0 for h < 0, the smallest
+    // subnormal for h = 0; an infinite or NaN h leaves every finite distance
+    // in range.
     float s2 = hh * hh;
-    if (hh > 0.0f && std::isfinite(hh)) {
+    if (std::isfinite(hh)) {
         while (s2 > 0.0f && std::sqrt(s2) > hh) s2 = std::nextafter(s2, 0.0f);
         while (!(std::sqrt(s2) > hh)) s2 = std::nextafter(s2, INFINITY);
     } else {
