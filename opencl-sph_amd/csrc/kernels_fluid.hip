@@ -441,6 +441,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
+    const float r2o = p.r2_out;   // read once: with a scene table p is in global memory
     const float *__restrict__ pos = f.position;
     const float *__restrict__ vel = f.velocity;
     const float *__restrict__ rho = f.density;
@@ -452,7 +453,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
     for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
         if (j == i) return;
         const V3 xj = load3(pos, j);
-        if (length(xj - xi) > kc.h) return;
+        if (sqlength(xj - xi) >= r2o) return;   // length > h, without the square root
         const float rhoj = rho[j];
         acc.pair(visc, kc, xi, vi, rhoi, pi, xj, load3(vel, j), rhoj, pressure(p, rhoj));
     });
@@ -468,6 +469,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
+    const float r2o = p.r2_out;   // read once: with a scene table p is in global memory
     const float *__restrict__ pos = f.position;
     const float *__restrict__ vel = f.velocity;
     const float *__restrict__ rho = f.density;
@@ -494,7 +496,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
         for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
             if (j == i) return;
             const V3 xj = load3(pos, j);
-            if (length(xj - xi) > kc.h) return;
+            if (sqlength(xj - xi) >= r2o) return;   // length > h, without the square root
             acc.pair(visc, kc, xi, vi, rhoi, pi, xj, load3(vel, j), rho[j], prs[j]);
         });
     }

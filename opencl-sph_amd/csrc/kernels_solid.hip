@@ -144,6 +144,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
     if (skip_lane<MODE>(cnt)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
+    const float r2o = p.r2_out;   // read once: with a scene table p is in global memory
     const float *__restrict__ pos = f.position;
     const float *__restrict__ opos = f.originalpos;
     const V3 xi = load3(pos, i), x0i = load3(opos, i);
@@ -169,8 +170,8 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 pv = xj - xi;
-            const float r = length(pv);
-            if (!(r > kc.h)) add_apq(apq, pv, x0j - x0i, w_poly6(r, kc));
+            const float s = sqlength(pv);   // r <= h as s < r2_out: no square root out of range
+            if (!(s >= r2o)) add_apq(apq, pv, x0j - x0i, w_poly6(sqrtf(s), kc));
         }
         if (inear) {
             const unsigned ok = planes_near_j(p, xj, inear);
@@ -211,7 +212,7 @@ __global__ void __launch_bounds__(kBlock) k_rotations_strains(PS ps, FieldPtrs f
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 q = x0j - x0i;
-            if (!(length(q) > kc.h)) {
+            if (!(sqlength(q) >= r2o)) {
                 const V3 u = mat_mul(rt, xj - xi) - q;
                 add_outer(du, div_recip(u * mass, rd0i), grad_spiky(-q, kc));
             }
@@ -268,6 +269,7 @@ __global__ void __launch_bounds__(kBlock) k_deformations_svk(PS ps, FieldPtrs f,
     if (skip_lane<MODE>(cnt)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
+    const float r2o = p.r2_out;   // read once: with a scene table p is in global memory
     const float *__restrict__ pos = f.position;
     const float *__restrict__ opos = f.originalpos;
     const V3 xi = load3(pos, i), x0i = load3(opos, i);
@@ -296,10 +298,10 @@ __global__ void __launch_bounds__(kBlock) k_deformations_svk(PS ps, FieldPtrs f,
         const V3 xj = load3(pos, j), x0j = load3(opos, j);
         if (j != i) {
             const V3 pv = xj - xi;
-            const float r = length(pv);
-            if (!(r > kc.h)) {
+            const float s = sqlength(pv);
+            if (!(s >= r2o)) {
                 const V3 q = x0j - x0i;
-                const float w = w_poly6(r, kc);
+                const float w = w_poly6(sqrtf(s), kc);
                 add_apq(apq, pv, q, w);
                 add_aqq(aqq, q, w);
             }
@@ -381,6 +383,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, Pa
     if (skip_lane<MODE>(cnt)) return;
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
+    const float r2o = p.r2_out;   // read once: with a scene table p is in global memory
     const float *__restrict__ pos = f.position;
     const float *__restrict__ opos = f.originalpos;
     const float *__restrict__ vel = f.velocity;
@@ -448,9 +451,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, Pa
         const V3 x0j = load3(opos, j);
         const V3 xj = load3(pos, j);
         const V3 X = x0i - x0j;
-        const float rx = length(X);
-        const float rcur = length(xj - xi);
-        const bool near_pair = !(rx > kc.h) || !(rcur > kc.h);
+        const bool near_pair = !(sqlength(X) >= r2o) || !(sqlength(xj - xi) >= r2o);
         if (!near_pair && !inear) return;
         const V3 vj = load3(vel, j);
         const float vivj = mass * mass / d0i / f.density0[j];
