@@ -148,3 +148,26 @@ def test_served_stream_same_alone_and_in_mix():
     assert sorted(got) == sorted(want) == list(range(len(reqs)))
     for r in reqs:
         check_same(f"request {r.index} ({r.kind}, {r.n})", got[r.index], want[r.index])
+
+
+def test_served_results_equal_oracle():
+    """Requests through the continuous-batching loop (batched steps, graph
+    replay, the benchmarks' Kahan density sums and FP64 polar) against the
+    FP32 oracle stepping each request's scene with the same settings: bit for
+    bit."""
+    from gpu_util import oracle_state
+    from sph import serve
+    reqs = serve.make_requests(count=9, rate=400.0, seed=11, nmin=300, nmax=2500, steps=10)
+    say("served vs oracle: " + ", ".join(f"{r.kind}:{r.n}" for r in reqs))
+    got = {}
+    fields = ("position", "velocity")
+    serve.serve_batched(reqs, slots=1 << 16, cells=1 << 18, scene_entries=16, graph_steps=4, realtime=False,
+                        compensated_sums=1, fields=fields, on_result=lambda r, res: got.__setitem__(r.index, res))
+    assert sorted(got) == list(range(len(reqs)))
+    for r in reqs:
+        scene = r.scene()
+        st = oracle_state(scene, params={"compensated": 1})
+        st.step(scene.system, r.steps)
+        say(f"request {r.index} ({r.kind}, {r.n}):")
+        for k in fields:
+            assert compare(k, got[r.index][k], st.f[k][:scene.n]) == 0
