@@ -157,7 +157,8 @@ def test_served_results_equal_oracle():
     bit."""
     from gpu_util import oracle_state
     from sph import serve
-    reqs = serve.make_requests(count=9, rate=400.0, seed=11, nmin=300, nmax=2500, steps=10)
+    # seed 41: 3 fluid, 4 corotational and 3 SVK requests
+    reqs = serve.make_requests(count=10, rate=400.0, seed=41, nmin=300, nmax=2500, steps=10)
     say("served vs oracle: " + ", ".join(f"{r.kind}:{r.n}" for r in reqs))
     got = {}
     fields = ("position", "velocity")
