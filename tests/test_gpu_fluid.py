@@ -122,3 +122,13 @@ def test_kahan_sums_bitwise(hip_device, steps, compensated):
     say(f"fluid {steps} step(s), compensated={compensated}, n={scene.n}")
     got, st = run_pair(scene, steps, compensated=compensated)
     check(got, st)
+
+
+def test_large_dam_break_three_steps_bitwise(hip_device):
+    """The 4.19 M-particle benchmark scene (Tait, mass-consistent, Kahan
+    density sums as benchmarked): three full steps (binning and reorder of an
+    already sorted state included) bitwise against the oracle."""
+    scene = scenes.baseline_config(3)
+    say(f"{scene.name}: n={scene.n}, 3 steps, Kahan density, bitwise")
+    got, st = run_pair(scene, 3, compensated=1)
+    check(got, st)
