@@ -123,3 +123,18 @@ def test_kahan_sums_bitwise(hip_device, kind, compensated):
     got, st = run_both(kind, 5, compensated=compensated)
     for k, v in got.items():
         compare(k, v, st.f[k].reshape(v.shape))
+
+
+def test_svk_cube_262k_bitwise(hip_device):
+    """The 262 k-particle SVK cube (seeded smooth velocity field), three full
+    steps with Kahan density sums, bitwise against the oracle."""
+    scene = scenes.baseline_config(2)
+    say(f"{scene.name}: n={scene.n}, 3 steps, Kahan density, bitwise")
+    s = engine(scene, compensated=1)
+    st = oracle_state(scene, params={"compensated": 1})
+    s.run(3)
+    st.step("svk", 3)
+    got = download(s, "position", "velocity", "density", "deformation", "stress", "force")
+    for k, v in got.items():
+        compare(k, v, st.f[k].reshape(v.shape))
+    s.free()
