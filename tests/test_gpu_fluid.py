@@ -132,3 +132,17 @@ def test_large_dam_break_three_steps_bitwise(hip_device):
     say(f"{scene.name}: n={scene.n}, 3 steps, Kahan density, bitwise")
     got, st = run_pair(scene, 3, compensated=1)
     check(got, st)
+
+
+@pytest.mark.parametrize("compensated", [0, 3], ids=["plain", "kahan_all"])
+def test_overflowing_lists_bitwise(hip_device, compensated):
+    """Packed at dx = h/2.35, most interior particles have more than 48
+    neighbours within h, more than the density pass's neighbour list holds:
+    those lanes take the candidate scan in the density and force passes
+    while the surface lanes read their lists.  Five steps, bitwise."""
+    scene = scenes.dam_break(20, 20, 10, 0.04 / 2.35, 0.04, 2e-3, seed=11)
+    say(f"fluid dense packing, 5 steps, compensated={compensated}, n={scene.n}")
+    got, st = run_pair(scene, 5, compensated=compensated)
+    check(got, st)
+    nn = st.num_neighbours
+    assert (nn > 48).mean() > 0.25 and (nn < 47).mean() > 0.25

@@ -17,10 +17,10 @@ from sph import scenes
 pytestmark = pytest.mark.gpu
 
 
-def small_solid(kind, nside=9, stretch=1.01, planes=True, constraints=True, bodies=True, seed=4):
+def small_solid(kind, nside=9, stretch=1.01, planes=True, constraints=True, bodies=True, seed=4, h=0.02):
     """A cube strained by 1% along x, with mirror planes through its low x
     and y faces, a plane constraint on its high-x face and a clamped layer."""
-    dx, h = 0.01, 0.02
+    dx = 0.01
     pts, g = scenes.lattice(nside, nside, nside, dx)
     rng = np.random.Generator(np.random.PCG64(seed))
     n = len(pts)
@@ -138,3 +138,18 @@ def test_svk_cube_262k_bitwise(hip_device):
     for k, v in got.items():
         compare(k, v, st.f[k].reshape(v.shape))
     s.free()
+
+
+@pytest.mark.parametrize("kind", ["corotational", "svk"])
+@pytest.mark.parametrize("compensated", [0, 3], ids=["plain", "kahan_all"])
+def test_overflowing_lists_bitwise(hip_device, kind, compensated):
+    """h = 2.35 dx: most interior particles have more than 48 neighbours
+    within h, more than a neighbour list holds, so their lanes scan the
+    candidates in every solid pass while the surface lanes read their
+    lists.  Three steps, bitwise."""
+    say(f"{kind}: dense packing, 3 steps, compensated={compensated}, bitwise")
+    got, st = run_both(kind, 3, compensated=compensated, nside=12, h=0.0235, planes=False)
+    for k, v in got.items():
+        compare(k, v, st.f[k].reshape(v.shape))
+    nn = st.num_neighbours
+    assert (nn > 48).mean() > 0.25 and (nn < 47).mean() > 0.1
