@@ -92,6 +92,32 @@ def test_batch_matches_scenes_alone(graph_steps, compensated):
         check_same(k, got[k], want[k])
 
 
+@pytest.mark.parametrize("compensated", [0, 3])
+def test_batch_dense_packing_matches_alone(compensated):
+    """Scenes packed at h = 2.35 dx overflow most interior neighbour lists:
+    the batched passes take the candidate-scan fallback on the same lanes as
+    the single-scene passes, with the same results, bit for bit."""
+    from sph.batch import Batch
+    sc = {"fD": scenes.dam_break(12, 12, 8, 0.04 / 2.35, 0.04, 2e-3, seed=3, steps=6, name="fD"),
+          "cD": scenes.cantilever(12, 8, 8, h=0.0235, steps=5, name="cD"),
+          "sD": scenes.svk_cube(nside=10, h=0.0235, steps=4, name="sD")}
+    say(f"batch dense packing: compensated={compensated}")
+    want = {k: alone(s, compensated) for k, s in sc.items()}
+    batch = Batch(slots=1 << 14, cells=1 << 16, scenes=16, compensated_sums=compensated, graph_steps=4)
+    try:
+        h = {k: batch.add(s) for k, s in sc.items()}
+        batch.step(6)
+        got = {}
+        for k, s in sc.items():
+            assert batch.steps_left(h[k]) == 0, k
+            got[k] = batch.read(h[k], *FIELDS[s.system])
+            batch.remove(h[k])
+    finally:
+        batch.free()
+    for k in sc:
+        check_same(k, got[k], want[k])
+
+
 def test_batch_admission_when_full():
     """A scene that does not fit is refused (None) and fits after removals."""
     from sph.batch import Batch
