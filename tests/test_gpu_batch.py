@@ -94,13 +94,17 @@ def test_batch_matches_scenes_alone(graph_steps, compensated):
 
 @pytest.mark.parametrize("compensated", [0, 3])
 def test_batch_dense_packing_matches_alone(compensated):
-    """Scenes packed at h = 2.35 dx overflow most interior neighbour lists:
-    the batched passes take the candidate-scan fallback on the same lanes as
-    the single-scene passes, with the same results, bit for bit."""
+    """Scenes packed at h = 2.35 dx overflow most interior neighbour lists,
+    and a crowded cell overflows the list entries' row offsets: the batched
+    passes take the candidate-scan fallback on the same lanes as the
+    single-scene passes, with the same results, bit for bit."""
     from sph.batch import Batch
     sc = {"fD": scenes.dam_break(12, 12, 8, 0.04 / 2.35, 0.04, 2e-3, seed=3, steps=6, name="fD"),
           "cD": scenes.cantilever(12, 8, 8, h=0.0235, steps=5, name="cD"),
-          "sD": scenes.svk_cube(nside=10, h=0.0235, steps=4, name="sD")}
+          "sD": scenes.svk_cube(nside=10, h=0.0235, steps=4, name="sD"),
+          # 4913 particles in one cell: rows longer than a list entry addresses
+          "fC": scenes.dam_break(17, 17, 17, 0.04 / 17, 0.04, 1e-4, seed=5, origin=(0.481, 0.481, 0.481), steps=3,
+                                 name="fC")}
     say(f"batch dense packing: compensated={compensated}")
     want = {k: alone(s, compensated) for k, s in sc.items()}
     batch = Batch(slots=1 << 14, cells=1 << 16, scenes=16, compensated_sums=compensated, graph_steps=4)

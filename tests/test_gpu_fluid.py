@@ -146,3 +146,16 @@ def test_overflowing_lists_bitwise(hip_device, compensated):
     check(got, st)
     nn = st.num_neighbours
     assert (nn > 48).mean() > 0.25 and (nn < 47).mean() > 0.25
+
+
+def test_crowded_row_takes_the_scan(hip_device):
+    """4913 particles in one grid cell: each neighbour row holds more slots
+    than a 12-bit list entry addresses (4096), so every lane takes the
+    candidate scan in the density and force passes.  Two steps, Kahan
+    density sums, bitwise."""
+    scene = scenes.dam_break(17, 17, 17, 0.04 / 17, 0.04, 1e-4, seed=5, origin=(0.481, 0.481, 0.481),
+                             name="crowded")
+    say(f"fluid crowded cell, 2 steps, n={scene.n}")
+    got, st = run_pair(scene, 2, compensated=1)
+    check(got, st)
+    assert st.num_neighbours.max() > 4096

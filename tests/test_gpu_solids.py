@@ -153,3 +153,26 @@ def test_overflowing_lists_bitwise(hip_device, kind, compensated):
         compare(k, v, st.f[k].reshape(v.shape))
     nn = st.num_neighbours
     assert (nn > 48).mean() > 0.25 and (nn < 47).mean() > 0.1
+
+
+@pytest.mark.parametrize("kind", ["corotational", "svk"])
+def test_crowded_rows_take_the_scan(hip_device, kind):
+    """4913 particles within one smoothing length, over 8 grid cells: each
+    neighbour row holds more than 1024 slots, more than a solid list entry
+    addresses, so every lane scans the candidates in every solid pass.  Two
+    steps, Kahan density sums, bitwise."""
+    if kind == "corotational":
+        scene = scenes.cantilever(17, 17, 17, dx=0.001, h=0.02, steps=2, name="crowded")
+    else:
+        scene = scenes.svk_cube(nside=17, dx=0.001, h=0.02, steps=2, name="crowded")
+    say(f"{kind}: crowded rows, 2 steps, n={scene.n}")
+    s = engine(scene, compensated=1)
+    st = oracle_state(scene, params={"compensated": 1})
+    s.run(2)
+    st.step(kind, 2)
+    names = ["position", "velocity", "density", "stress", "force"]
+    names += ["rotation"] if kind == "corotational" else ["deformation"]
+    got = download(s, *names)
+    s.free()
+    for k, v in got.items():
+        compare(k, v, st.f[k].reshape(v.shape))
