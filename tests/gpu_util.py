@@ -49,10 +49,17 @@ def download(system, *names):
 
 
 def compare(name, got, want, rtol=0.0, atol=0.0):
-    """Max abs/rel difference and mismatch count; asserts within tolerance."""
+    """Max abs/rel difference and mismatch count; asserts within tolerance.
+
+    Equal means the same bit pattern (so -0.0 differs from +0.0); any NaN
+    equals any NaN, whose payload bits are not specified."""
     got = np.asarray(got).reshape(np.asarray(want).shape)
     want = np.asarray(want)
-    if got.dtype.kind == "f":
+    if got.dtype.kind == "f" and got.dtype == want.dtype:
+        bits = np.dtype(f"u{got.dtype.itemsize}")
+        same = (np.ascontiguousarray(got).view(bits) == np.ascontiguousarray(want).view(bits))
+        same |= np.isnan(got) & np.isnan(want)
+    elif got.dtype.kind == "f":
         same = (got == want) | (np.isnan(got) & np.isnan(want))
     else:
         same = got == want
@@ -61,7 +68,7 @@ def compare(name, got, want, rtol=0.0, atol=0.0):
     maxabs = float(np.nanmax(diff)) if diff.size else 0.0
     scale = np.abs(want.astype(np.float64))
     tol = atol + rtol * scale
-    bad = ~same & ~(diff <= tol)
+    bad = ~same if rtol == 0.0 and atol == 0.0 else ~same & ~(diff <= tol)
     say(f"  {name}: {mism}/{want.size} differ bitwise, max |diff| {maxabs:.3e}, outside tolerance {int(bad.sum())}")
     assert not bad.any(), f"{name}: {int(bad.sum())} entries outside rtol={rtol} atol={atol} (max |diff| {maxabs})"
     return mism
