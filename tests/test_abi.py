@@ -47,3 +47,16 @@ def test_package_imports_and_reports_errors_without_device():
         pass
     else:
         raise AssertionError("empty specification must raise")
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without the HIP library the package refuses to import: there is no CPU
+    fallback for the simulation step."""
+    import subprocess
+    import sys
+    env = dict(os.environ, SPH_HIP_LIBRARY=str(tmp_path / "absent" / "libsph_hip.so"),
+               PYTHONPATH=os.path.join(REPO, "opencl-sph_amd"))
+    r = subprocess.run([sys.executable, "-c", "import sph.bindings"], env=env, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "ImportError" in r.stderr and "no CPU fallback" in r.stderr
