@@ -2,14 +2,19 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload served_stream]
 
-Default workload: the served scene stream -- seeded requests (50% fluid,
-30% corotational, 20% SVK blocks, 1k-262k particles log-uniform, K = 200
-steps each) run through the device batch (continuous batching, hipGraph
-step replay, HBM-budget admission) and, for comparison, one scene at a
-time through the single-scene path.  `value` is the batch's throughput with
-every request queued at the start (saturated); the same line reports the
-single-scene path on a prefix of the stream, the batch on that prefix, and
-p50/p99 request latency with Poisson arrivals at --rate per second.
+Default workload: the served scene stream -- 2048 seeded requests (50%
+fluid, 30% corotational, 20% SVK blocks, 1k-262k particles log-uniform, 200
+steps each, --request-steps) run through the device batch (continuous
+batching, hipGraph step replay, HBM-budget admission) and, for comparison,
+one scene at a time through the single-scene path.  `value` is the batch's
+throughput with every request queued at the start (saturated); a "step" of
+the line is one batch step (every resident scene advances one step), so
+`steps` x `ms_per_step` is the saturated run's wall time.  --steps and
+--warmup do not change the stream: --warmup is the number of warm-up
+requests (of another seed).  The same line reports the single-scene path
+on a prefix of the stream, the batch on that prefix, p50/p99 request
+latency with Poisson arrivals at --rate per second, and the 4.19M-particle
+dam break (50 steps, plain single-scene path) with its step roofline.
 Scenes, results and precision settings are identical between the two paths
 (bitwise, as the GPU tests check), so the comparison is at fixed error.
 
@@ -127,11 +132,31 @@ def copy_probe(mib):
     log(f"copy probe: 3 x {mib} MiB device-to-device")
 
 
-def cpu_baseline(scenes_, budget_s):
-    """The FP32 oracle on the host cores over the given scenes (reference
-    configuration set up for solids, one warm-up step each), as many whole
-    steps as fit an equal share of the time budget."""
+def host_description():
+    """nproc, the CPU set this process may use, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
+
+
+def cpu_baseline(scenes_, budget_s, compensated=1, repeats=5):
+    """The FP32 oracle on the host cores, per system: after one warm-up step,
+    `repeats` timed repeats of k steps each (k sized so that all repeats of
+    all systems fit `budget_s`); the median repeat gives the system's rate.
+    The overall value is total particle-steps over the summed medians."""
     import ctypes
+    import statistics
 
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
@@ -139,8 +164,9 @@ def cpu_baseline(scenes_, budget_s):
     lib, _, _ = oracle.load("f32")
     lib.orc_threads.restype = ctypes.c_uint
     threads = int(lib.orc_threads())
-    work = elapsed = 0.0
-    parts = []
+    per_system = {}
+    work = med_total = 0.0
+    share = budget_s / max(len(scenes_), 1) / repeats
     for scene in scenes_:
         fields = {k: v for k, v in scene.fields.items() if k in oracle.OracleState.PER_PARTICLE}
         for k in ("body_number", "body_dynamic"):
@@ -148,30 +174,130 @@ def cpu_baseline(scenes_, budget_s):
                 fields[k] = scene.fields[k]
         params = dict(scene.params)
         params["polar_fp64"] = 1
+        params["compensated"] = compensated
         st = oracle.OracleState("f32", params, **fields)
         if scene.system in ("corotational", "svk"):
             st.setup_solid()
-        st.step(scene.system, 1)                    # warm-up
-        steps, t0 = 0, time.perf_counter()
-        while True:
-            st.step(scene.system, 1)
-            steps += 1
-            el = time.perf_counter() - t0
-            if el >= budget_s / len(scenes_) or steps >= scene.steps:
-                break
-        work += scene.n * steps
-        elapsed += el
-        parts.append(f"{steps} step(s) of {scene.name} ({scene.system}, {scene.n} particles)")
-    return {"value": work / elapsed, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": "; ".join(parts) + f"; FP32 oracle (C restatement of the reference kernels, OpenMP), "
-                      f"after 1 warm-up step each, {elapsed:.1f} s wall"}
+        t0 = time.perf_counter()
+        st.step(scene.system, 1)                    # warm-up, also sizes a repeat
+        one = time.perf_counter() - t0
+        k = max(1, min(int(share / max(one, 1e-6)), scene.steps))
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            st.step(scene.system, k)
+            times.append(time.perf_counter() - t0)
+        med = statistics.median(times)
+        per_system[scene.system] = {"value": scene.n * k / med, "scene": scene.name, "particles": scene.n,
+                                    "steps_per_repeat": k, "median_s": round(med, 4),
+                                    "repeats_s": [round(t, 4) for t in times]}
+        work += scene.n * k
+        med_total += med
+    host = host_description()
+    return {"value": work / med_total, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "; ".join(f"{v['scene']} ({k}, {v['particles']} particles): {v['steps_per_repeat']} steps x "
+                                f"{repeats} repeats, median {v['median_s']} s" for k, v in per_system.items())
+                      + f"; FP32 oracle (C restatement of the reference kernels, OpenMP, compensated sums "
+                        f"{compensated}, FP64 polar), after 1 warm-up step",
+            "per_system": per_system, "repeats": repeats, "threads": threads, **host,
+            "reference_cpu_opencl": "unavailable: the reference creates a GPU-only OpenCL context "
+                                    "(src/opencl/particle_system_host.c:65-88) and its solid programs do not "
+                                    "compile under clang-based OpenCL; no CPU OpenCL device is installed"}
+
+
+def pass_roofline(name, n_launch, ms, nbytes, flops, bound):
+    """Roofline entry of one pass from its total time over n_launch launches
+    and its algorithmic bytes / flops per launch.  `bound` names the peak the
+    fraction is taken against ("hbm" or "valu"); both rates are reported."""
+    from sph import roofline
+    avg_s = ms / n_launch / 1e3
+    out = {"avg_launch_ms": round(avg_s * 1e3, 5), "bound": bound}
+    if nbytes is not None:
+        out["bytes_per_launch"] = int(nbytes)
+        out["achieved_gbs"] = round(nbytes / avg_s / 1e9, 2)
+        out["hbm_frac"] = round(nbytes / avg_s / 1e9 / roofline.MI355X_HBM_GBS, 5)
+    if flops:
+        out["flops_per_launch"] = int(flops)
+        out["achieved_tflops"] = round(flops / avg_s / 1e12, 3)
+        out["valu_frac"] = round(flops / avg_s / 1e12 / roofline.MI355X_FP32_TFLOPS, 5)
+    out["frac"] = out.get("valu_frac" if bound == "valu" else "hbm_frac")
+    return out
+
+
+def roofline_block(kernel, entry, traffic, issue, sample=None):
+    """The line's `roofline` object for the dominant kernel."""
+    from sph import roofline
+    valu = entry["bound"] == "valu"
+    block = {"kernel": kernel, "bound": entry["bound"],
+             "achieved": entry["achieved_tflops"] if valu else entry["achieved_gbs"],
+             "peak": roofline.MI355X_FP32_TFLOPS if valu else roofline.MI355X_HBM_GBS,
+             "unit": "TFLOP/s" if valu else "GB/s", "frac": entry["frac"], "traffic": traffic,
+             "avg_launch_ms": entry["avg_launch_ms"], "bytes_per_launch": entry.get("bytes_per_launch"),
+             "flops_per_launch": entry.get("flops_per_launch"), "hbm_frac": entry.get("hbm_frac"),
+             "valu_frac": entry.get("valu_frac"), "valu_issue": issue}
+    if sample:
+        block["sample"] = sample
+    return block
+
+
+def measure_single(scene, steps, warmup, sums, tuning):
+    """K timed steps of one scene on the single-scene path (state resident in
+    HBM), then K profiled steps with per-pass HIP-event timing; returns the
+    measurements and, for fluids, the step roofline with every pass's
+    fraction of the bound that applies to it."""
+    import numpy as np
+
+    from sph import autotune, roofline, scenes
+
+    autotune.apply(tuning)
+    log(f"workload {scene.name}: {scene.n} particles, system {scene.system}")
+    system = scenes.build(scene)
+    handle = system.psdata_opencl
+    handle.set_precision(compensated_sums=SUMS[sums], polar_fp64=True)
+
+    system.run(warmup)
+    handle.synchronize()
+    log(f"warm-up {warmup} steps done")
+    t0 = time.perf_counter()
+    system.run(steps)
+    handle.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / steps * 1e3
+    value = scene.n * steps / elapsed
+    log(f"timed {steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} particle-steps/s")
+
+    handle.reset_kernel_stats()
+    handle.set_profiling(True)
+    system.run(steps)
+    handle.synchronize()
+    stats = handle.kernel_stats()
+    handle.set_profiling(False)
+    per_step = {k: ms / steps for k, (launches, ms) in stats.items()}
+    log("per-kernel ms/step: " + ", ".join(f"{k}={v:.4f}" for k, v in sorted(per_step.items(), key=lambda x: -x[1])))
+    cells = int(np.prod(system.get_fields("gridcount")[0].shape))
+    out = {"scene": scene, "value": value, "ms_per_step": ms_per_step, "steps": steps, "cells": cells,
+           "stats": stats, "per_step": per_step}
+    if scene.system == "fluid":
+        candidates, in_range = grid_pair_counts(system, scene)
+        passes = {}
+        for k, (n_launch, ms) in stats.items():
+            nbytes = roofline.bytes_per_launch(k, scene.n, cells)
+            flops = roofline.flops_per_launch(k, candidates, in_range)
+            if nbytes is None:
+                continue
+            passes[k] = pass_roofline(k, n_launch, ms, nbytes, flops, "valu" if flops else "hbm")
+        t_roof = roofline.t_roof_seconds(scene.n, cells, candidates, in_range)
+        out["passes"] = passes
+        out["step_roofline"] = {"t_roof_ms": round(t_roof * 1e3, 5), "frac": round(t_roof * 1e3 / ms_per_step, 4),
+                                "algorithmic_bytes_per_step": roofline.step_bytes(scene.n, cells),
+                                "candidate_pairs": candidates, "in_range_pairs": in_range,
+                                "rule": "sum over passes of max(bytes / 8 TB/s, flops / 157.3 TFLOP/s)"}
+    system.free()
+    return out
 
 
 def run_single(args):
-    import numpy as np
-
-    import sph
-    from sph import autotune, roofline, scenes
+    from sph import autotune, scenes
 
     scene = scenes.baseline_config(WORKLOADS[args.workload])
     if args.autotune == "run":
@@ -180,90 +306,48 @@ def run_single(args):
         tuning = autotune.lookup(autotune.single_signature(scene))
     else:
         tuning = dict(autotune.DEFAULTS)
-    autotune.apply(tuning)
-    log(f"workload {scene.name}: {scene.n} particles, system {scene.system}")
-    system = scenes.build(scene)
-    handle = system.psdata_opencl
-    handle.set_precision(compensated_sums=SUMS[args.sums], polar_fp64=True)
-
-    system.run(args.warmup)
-    handle.synchronize()
-    log(f"warm-up {args.warmup} steps done")
-
-    t0 = time.perf_counter()
-    system.run(args.steps)
-    handle.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1e3
-    value = scene.n * args.steps / elapsed
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} particle-steps/s")
-
-    handle.reset_kernel_stats()
-    handle.set_profiling(True)
-    system.run(args.steps)
-    handle.synchronize()
-    stats = handle.kernel_stats()
-    handle.set_profiling(False)
-    per_step = {k: ms / args.steps for k, (launches, ms) in stats.items()}
-    log("per-kernel ms/step: " + ", ".join(f"{k}={v:.4f}" for k, v in sorted(per_step.items(), key=lambda x: -x[1])))
-
-    cells = int(np.prod(system.get_fields("gridcount")[0].shape))
+    m = measure_single(scene, args.steps, args.warmup, args.sums, tuning)
+    stats = m["stats"]
     dominant = max(stats, key=lambda k: stats[k][1])
-    launches, total_ms = stats[dominant]
-    avg_s = total_ms / launches / 1e3
-    bytes_launch = roofline.bytes_per_launch(dominant, scene.n, cells)
     roof = None
-    if bytes_launch is not None:
-        achieved = bytes_launch / avg_s / 1e9
-        roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.MI355X_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / roofline.MI355X_HBM_GBS, 4),
-                "traffic": measured_traffic(scene.name, dominant),
-                "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_s * 1e3, 5),
-                "valu_issue": measured_issue(scene.name, dominant)}
-    extra = {}
-    if scene.system == "fluid":
-        candidates, in_range = grid_pair_counts(system, scene)
-        flops = roofline.flops_per_launch(dominant, candidates, in_range)
-        if roof is not None and flops:
-            roof["valu_tflops"] = round(flops / avg_s / 1e12, 3)
-            roof["valu_frac"] = round(flops / avg_s / 1e12 / roofline.MI355X_FP32_TFLOPS, 4)
-        t_roof = roofline.t_roof_seconds(scene.n, cells, candidates, in_range)
-        # the streaming (HBM-bound) kernels against the HBM peak, each on its
-        # own algorithmic bytes
-        streaming = {}
-        for k in ("bin_key_count", "bin_scan", "bin_scatter", "bin_rank_place", "step_forward"):
-            if k not in stats:
-                continue
-            n_launch, ms = stats[k]
-            b = roofline.bytes_per_launch(k, scene.n, cells)
-            gbs = b / (ms / n_launch / 1e3) / 1e9
-            streaming[k] = {"bytes_per_launch": b, "avg_launch_ms": round(ms / n_launch, 5),
-                            "achieved_gbs": round(gbs, 1), "frac": round(gbs / roofline.MI355X_HBM_GBS, 4)}
-        extra = {"step_roofline": {"t_roof_ms": round(t_roof * 1e3, 5), "frac": round(t_roof * 1e3 / ms_per_step, 4),
-                                   "algorithmic_bytes_per_step": roofline.step_bytes(scene.n, cells),
-                                   "candidate_pairs": candidates, "in_range_pairs": in_range},
-                 "hbm_kernels": streaming,
-                 "kernel_ms_per_step": {k: round(v, 5) for k, v in per_step.items()}}
-    system.free()
-
+    if dominant in m.get("passes", {}):
+        roof = roofline_block(dominant, m["passes"][dominant], measured_traffic(scene.name, dominant),
+                              measured_issue(scene.name, dominant))
     cpu = None
     if args.cpu_budget > 0:
         log("CPU baseline (FP32 oracle) ...")
-        cpu = cpu_baseline([scene], args.cpu_budget)
+        cpu = cpu_baseline([scene], args.cpu_budget, SUMS[args.sums])
         log(f"CPU baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} threads")
-
     line = {
-        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "metric": METRIC, "value": round(m["value"], 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(m["ms_per_step"], 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded lattice with uniform jitter)",
-        "config": {"workload": scene.name, "particles": scene.n, "cells": cells, "system": scene.system,
+        "config": {"workload": scene.name, "particles": scene.n, "cells": m["cells"], "system": scene.system,
                    "path": "plain", "sums": args.sums, "tuning": tuning,
                    "fluid_model": "tait eos, mass-consistent force" if scene.system == "fluid" else None},
         "roofline": roof, "cpu_baseline": cpu,
+        "kernel_ms_per_step": {k: round(v, 5) for k, v in m["per_step"].items()},
     }
-    line.update(extra)
+    if "step_roofline" in m:
+        line["step_roofline"] = m["step_roofline"]
+        line["passes"] = m["passes"]
     return line
+
+
+def dam_break_4m_block(args):
+    """The 4.19M-particle dam break on the plain single-scene path, 50 timed
+    steps (its own step count), with the step roofline and every pass's
+    fraction of its bound."""
+    from sph import autotune, scenes
+    scene = scenes.baseline_config(3)
+    tuning = autotune.lookup(autotune.single_signature(scene)) if args.autotune != "off" else dict(autotune.DEFAULTS)
+    m = measure_single(scene, scene.steps, 5, args.sums, tuning)
+    return {"workload": scene.name, "particles": scene.n, "cells": m["cells"], "steps": m["steps"],
+            "ms_per_step": round(m["ms_per_step"], 5), "particle_steps_per_s": round(m["value"], 1),
+            "path": "plain (single scene)", "sums": args.sums, "tuning": tuning,
+            "step_roofline": m["step_roofline"], "passes": m["passes"],
+            "traffic_per_launch": {k: measured_traffic(scene.name, k) for k in m["passes"]}}
 
 
 def _brief(rep):
@@ -273,10 +357,39 @@ def _brief(rep):
     return {k: (round(v, 5) if isinstance(v, float) else v) for k, v in rep.items() if k in keep}
 
 
+def scene_pair_counts(scene):
+    """(candidate pairs, in-range pairs with length <= h, self included) of
+    a scene's initial state: candidates from the 27 cells around each
+    particle's cell of the reference's grid (cell edge |hi - lo| / res >= h),
+    in-range pairs from a k-d tree.  Solids move by a small fraction of h
+    over a request, so these are also the pairs of each of its steps."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+
+    pos = np.asarray(scene.fields["position"], np.float32)[:scene.n]
+    gb = np.asarray(scene.params["gridbounds"], np.float32).reshape(3, 2)
+    h = np.float32(scene.params["h"])
+    res = np.array([int(abs(float(gb[a, 1] - gb[a, 0])) / float(h)) for a in range(3)])
+    v = (pos - gb[:, 0]) / (gb[:, 1] - gb[:, 0]) * res.astype(np.float32)
+    cell = np.clip(np.where(v > 0, v, 0).astype(np.int64), 0, res - 1)
+    key = (cell[:, 2] * res[1] + cell[:, 1]) * res[0] + cell[:, 0]
+    count = np.bincount(key, minlength=int(np.prod(res))).reshape(res[2], res[1], res[0])
+    pad = np.pad(count, 1)
+    box = np.zeros_like(count)
+    for dz in range(3):
+        for dy in range(3):
+            for dx in range(3):
+                box += pad[dz:dz + res[2], dy:dy + res[1], dx:dx + res[0]]
+    candidates = int((count * box).sum())
+    tree = cKDTree(pos.astype(np.float64))
+    in_range = int(tree.count_neighbors(tree, float(h)))
+    return candidates, in_range
+
+
 def run_served(args):
     from sph import autotune, roofline, serve
 
-    steps = args.steps if args.steps_set else 200
+    steps = args.request_steps
     if args.autotune == "run":
         # tuned on requests of another seed, not on the measured stream
         sample = serve.make_requests(count=64, rate=args.rate, seed=args.seed + 2, steps=60)
@@ -294,12 +407,12 @@ def run_served(args):
     budget = args.hbm_budget_gb * 1e9
     log(f"served stream: {len(reqs)} requests, {steps} steps each, "
         f"{sum(r.n * r.steps for r in reqs):.4g} particle-steps, last arrival {reqs[-1].arrival:.1f} s")
+    warm = serve.make_requests(count=max(args.warmup, 1), rate=args.rate, seed=args.seed + 1, steps=4)
 
     if args.counters_only:
         # Only the profiled pass, for hardware-counter runs under rocprofv3,
         # after a batched warm-up like the one of the full run, so that the
         # traced pass meets the device in the same state as the bench's own.
-        warm = serve.make_requests(count=max(args.warmup, 1), rate=args.rate, seed=args.seed + 1, steps=4)
         serve.serve_batched(warm, hbm_budget=budget, realtime=False, compensated_sums=sums,
                             graph_steps=args.graph_steps)
         prof = serve.serve_batched(reqs[:args.profile_requests], hbm_budget=budget, realtime=False,
@@ -309,7 +422,6 @@ def run_served(args):
                 "kernel_stats": prof["kernel_stats"]}
 
     # warm-up: both paths on a few requests of another seed
-    warm = serve.make_requests(count=max(args.warmup, 1), rate=args.rate, seed=args.seed + 1, steps=4)
     serve.serve_batched(warm, hbm_budget=budget, realtime=False, compensated_sums=sums, graph_steps=args.graph_steps)
     serve.serve_plain(warm[:2], realtime=False, compensated_sums=sums)
     log("warm-up done")
@@ -333,24 +445,34 @@ def run_served(args):
             f"p99 {live_plain['latency_p99_s'] * 1e3:.1f} ms")
 
     # profiled pass: per-pass HIP-event times on a prefix, without graphs
-    prof = serve.serve_batched(reqs[:args.profile_requests], hbm_budget=budget, realtime=False,
-                               compensated_sums=sums, profile=True)
+    sample = reqs[:args.profile_requests]
+    prof = serve.serve_batched(sample, hbm_budget=budget, realtime=False, compensated_sums=sums, profile=True)
     stats = prof["kernel_stats"]
+    pair_steps = {}
+    for r in sample:
+        c, p = scene_pair_counts(r.scene())
+        a, b = pair_steps.get(r.kind, (0, 0))
+        pair_steps[r.kind] = (a + c * r.steps, b + p * r.steps)
+    passes = {}
+    for k, (n_launch, ms) in stats.items():
+        if not n_launch:
+            continue
+        nbytes = roofline.batch_bytes_per_launch(k, prof["particle_steps_by_system"], n_launch)
+        flops = roofline.batch_flops(k, pair_steps, bool(sums & 1))
+        passes[k] = pass_roofline(k, n_launch, ms, nbytes, flops / n_launch if flops else None,
+                                  "valu" if flops else "hbm")
     dominant = max(stats, key=lambda k: stats[k][1])
-    launches, total_ms = stats[dominant]
-    avg_s = total_ms / launches / 1e3
-    bytes_launch = roofline.batch_bytes_per_launch(dominant, prof["particle_steps_by_system"], launches)
-    roof = None
-    if bytes_launch:
-        achieved = bytes_launch / avg_s / 1e9
-        roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.MI355X_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / roofline.MI355X_HBM_GBS, 5),
-                "traffic": measured_traffic("served_stream", dominant),
-                "valu_issue": measured_issue("served_stream", dominant),
-                "bytes_per_launch": round(bytes_launch), "avg_launch_ms": round(avg_s * 1e3, 5),
-                "sample": f"first {len(reqs[:args.profile_requests])} requests, saturated, no graphs"}
-    pass_ms = {k: round(ms / max(n, 1), 5) for k, (n, ms) in sorted(stats.items(), key=lambda x: -x[1][1])}
-    log("per-pass ms/launch: " + ", ".join(f"{k}={v}" for k, v in pass_ms.items()))
+    roof = roofline_block(dominant, passes[dominant], measured_traffic("served_stream", dominant),
+                          measured_issue("served_stream", dominant),
+                          sample=f"first {len(sample)} requests ({steps} steps each), saturated, no graphs; "
+                                 f"pair counts of the requests' initial states")
+    log("per-pass ms/launch: " + ", ".join(f"{k}={v['avg_launch_ms']}" for k, v in passes.items()))
+
+    big = None
+    if args.dam_break_4m:
+        log("4.19M-particle dam break, plain path ...")
+        big = dam_break_4m_block(args)
+        log(f"4.19M dam break: {big['ms_per_step']:.3f} ms/step, t_roof fraction {big['step_roofline']['frac']}")
 
     cpu = None
     if args.cpu_budget > 0:
@@ -359,25 +481,29 @@ def run_served(args):
         for kind in serve.KINDS:
             r = next(q for q in reqs if q.kind == kind)
             picks.append(r.scene())
-        cpu = cpu_baseline(picks, args.cpu_budget)
+        cpu = cpu_baseline(picks, args.cpu_budget, sums)
         log(f"CPU baseline: {cpu['value']:.4g} particle-steps/s on {cpu['cores']} threads")
 
     return {
         "metric": METRIC, "value": round(sat["particle_steps_per_s"], 1), "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": steps, "warmup": args.warmup, "ms_per_step": round(sat["wall_s"] / max(sat["batch_steps"], 1) * 1e3, 5),
+        "steps": sat["batch_steps"], "warmup": args.warmup,
+        "ms_per_step": round(sat["wall_s"] / max(sat["batch_steps"], 1) * 1e3, 5),
+        "step_unit": "one batch step of the saturated stream (every resident scene advances one step); "
+                     "steps x ms_per_step = the timed wall time",
+        "steps_arg": args.steps, "timed_wall_s": round(sat["wall_s"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": f"synthetic (seeded request stream, seed {args.seed}: blocks with seeded jitter / smooth velocity)",
         "config": {"workload": "served_stream", "requests": len(reqs), "rate_per_s": args.rate,
                    "mix": "50% fluid / 30% corotational / 20% svk", "particles": "log-uniform 1024-262144",
                    "steps_per_request": steps, "hbm_budget_gb": args.hbm_budget_gb, "sums": args.sums,
                    "polar": "fp64", "graph_steps": args.graph_steps, "pools": sat.get("pools", 1), "path": "batched",
-                   "tuning": tuning},
+                   "tuning": tuning, "warmup_requests": max(args.warmup, 1)},
         "batched_saturated": _brief(sat),
         "prefix": {"requests": len(prefix), "batched": _brief(sat_prefix), "plain": _brief(plain_prefix),
                    "speedup": round(sat_prefix["particle_steps_per_s"] / plain_prefix["particle_steps_per_s"], 3)},
         "served": {"batched": _brief(live), "plain_prefix": _brief(live_plain) if live_plain else None},
         "error_vs_single_scene": "bitwise identical (tests/test_gpu_batch.py)",
-        "roofline": roof, "pass_ms_per_launch": pass_ms, "cpu_baseline": cpu,
+        "roofline": roof, "passes": passes, "dam_break_4m": big, "cpu_baseline": cpu,
     }
 
 
@@ -385,8 +511,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="steps per request (served stream, default 200) or timed steps (single scene, default 50)")
-    ap.add_argument("--warmup", type=int, default=5)
+                    help="timed steps (single-scene workloads, default 50); the served stream is fixed and "
+                         "reports batch steps")
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="warm-up steps (single scene) or warm-up requests of another seed (served stream)")
+    ap.add_argument("--request-steps", type=int, default=200, help="served stream: steps per request")
+    ap.add_argument("--dam-break-4m", type=int, default=1,
+                    help="served stream: also measure the 4.19M-particle dam break on the plain path")
     ap.add_argument("--workload", default="served_stream", choices=sorted(WORKLOADS))
     ap.add_argument("--sums", default="kahan-density", choices=sorted(SUMS),
                     help="neighbour-sum precision (default: Kahan-compensated density sums)")

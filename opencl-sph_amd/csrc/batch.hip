@@ -146,10 +146,10 @@ __global__ void __launch_bounds__(kBlock) k_batch_rank_gather(SceneTable ps, con
     const unsigned system = ps.at(t).system;
     copy_words<3>(a.pos, b.pos, from, f);
     copy_words<3>(a.vel, b.vel, from, f);
+    copy_words<1>(a.body, b.body, from, f);   // static bodies are frozen in every system
     if (system != kFluid) {
         copy_words<3>(a.opos, b.opos, from, f);
         copy_words<1>(a.d0, b.d0, from, f);
-        copy_words<1>(a.body, b.body, from, f);
     }
     if (system == kCorotational) {
         copy_words<9>(a.rot, b.rot, from, f);

@@ -45,6 +45,40 @@ BATCH_PASSES = {
     "compute_forces_solids_svk": (("svk",), 84, 8),
     "step_forward": (("fluid", "corotational", "svk"), 72, 0),
 }
+# Batch passes: (systems the pass runs for, flops per candidate pair, flops
+# per in-range pair (self included), flops per in-range pair when Kahan
+# density sums are on).  The neighbour passes are bound by vector issue:
+# their fraction is taken against the FP32 vector peak.  Candidate pairs are
+# those of the 27 cells around each particle's cell; the pair arithmetic per
+# in-range pair: density 4 (+4 compensated), fluid force 31, rotations and
+# strains 24 (A_pq) + 52 (displacement gradient) over two candidate sweeps,
+# SVK deformations 24 (A_pq) + 15 (A_qq), solid forces 110.  The FP64 polar
+# decomposition / A_qq inverse per particle is not counted.
+BATCH_FLOPS = {
+    "compute_density_fluid": (("fluid",), 8, 4, 8),
+    "compute_forces_fluids": (("fluid",), 8, 31, 31),
+    "compute_density_solid": (("corotational", "svk"), 8, 4, 8),
+    "compute_rotations_and_strains": (("corotational",), 16, 76, 76),
+    "compute_deformations": (("svk",), 8, 39, 39),
+    "compute_forces_solids_corotational": (("corotational",), 8, 110, 110),
+    "compute_forces_solids_svk": (("svk",), 8, 110, 110),
+}
+
+
+def batch_flops(name, pair_steps_by_system, kahan_density):
+    """Algorithmic flops of batch pass `name` over a run, from each system's
+    candidate and in-range pair-steps {system: (candidates, in_range)}."""
+    m = BATCH_FLOPS.get(name)
+    if m is None:
+        return None
+    systems, per_cand, per_pair, per_pair_kahan = m
+    total = 0.0
+    for k in systems:
+        cand, pairs = pair_steps_by_system.get(k, (0, 0))
+        total += per_cand * cand + (per_pair_kahan if kahan_density else per_pair) * pairs
+    return total
+
+
 # gathered words per particle in binning: pos, vel, id (+ opos, density0,
 # body; + rotation, strain for corotational), read and written, + perm
 GATHER_BYTES = {"fluid": 2 * 7 * 4 + 4, "corotational": 2 * 27 * 4 + 4, "svk": 2 * 12 * 4 + 4}

@@ -67,7 +67,7 @@ def spec_text(lines):
 
 
 def fluid_spec(pnum, mass, dt, h, gridbounds, restdens, stiffness, viscosity, gravity=(0, 0, -9.81),
-               eos_tait=False, mass_consistent=False):
+               eos_tait=False, mass_consistent=False, body_count=0):
     lines = [
         ("pnum", "1", "unsigned int", pnum),
         ("n", "1", "unsigned int", None),
@@ -91,6 +91,12 @@ def fluid_spec(pnum, mass, dt, h, gridbounds, restdens, stiffness, viscosity, gr
         ("eos_tait", "1", "unsigned int", int(bool(eos_tait))),
         ("mass_consistent", "1", "unsigned int", int(bool(mass_consistent))),
     ]
+    if body_count:
+        # the body fields of the solid confs: step_forward freezes particles
+        # of static bodies in every system (kernels/common.cl:699-704)
+        lines += [("body_number", "pnum", "unsigned int", None),
+                  ("body_count", "1", "unsigned int", body_count),
+                  ("body_dynamic", "body_count", "unsigned int", None)]
     return spec_text(lines)
 
 
@@ -152,18 +158,23 @@ def lattice(nx, ny, nz, dx, origin=(0.0, 0.0, 0.0)):
 
 def dam_break(nx, ny, nz, dx, h, jitter, seed=0, gridbounds=(0, 1, 0, 1, 0, 1), restdens=1000.0,
               stiffness=2000.0, viscosity=0.1, dt=1e-4, steps=100, eos_tait=True, mass_consistent=True,
-              origin=None, name="dam_break"):
+              origin=None, static_floor=False, name="dam_break"):
+    """static_floor: the block's lowest lattice layer is a static body
+    (body_number 1, body_dynamic [0])."""
     rng = np.random.Generator(np.random.PCG64(seed))
     if origin is None:
         origin = (gridbounds[0] + h, gridbounds[2] + h, gridbounds[4] + h)
-    pts, _ = lattice(nx, ny, nz, dx, origin)
+    pts, g = lattice(nx, ny, nz, dx, origin)
     pts = pts + rng.uniform(-jitter, jitter, pts.shape)
     pos = pts.astype(np.float32)
     n = len(pos)
     mass = restdens * dx ** 3
     spec = fluid_spec(n, mass, dt, h, gridbounds, restdens, stiffness, viscosity, eos_tait=eos_tait,
-                      mass_consistent=mass_consistent)
+                      mass_consistent=mass_consistent, body_count=1 if static_floor else 0)
     fields = {"position": pos, "n": np.array([n], np.uint32)}
+    if static_floor:
+        fields["body_number"] = (g[:, 2] == 0).astype(np.uint32)
+        fields["body_dynamic"] = np.array([0], np.uint32)
     params = dict(n=n, gridbounds=list(gridbounds), h=h, mass=float(np.float32(mass)), dt=dt, restdens=restdens,
                   stiffness=stiffness, viscosity=viscosity, gravity=(0.0, 0.0, -9.81), eos_tait=int(eos_tait),
                   mass_consistent=int(mass_consistent))

@@ -125,7 +125,8 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
     pool as with `pools=1`.  Profiling uses one pool and waits for every
     preparation it reaches, so that a profiled pass admits the same scenes
     in the same rounds however fast the workers run.
-    on_result(request, {field: array}) receives each result."""
+    on_result(request, {field: array}) receives each result; `fields` names
+    the fields read back, for every request or per system ({kind: names})."""
     from .batch import Batch
 
     npool = 1 if profile else max(1, int(pools))
@@ -217,7 +218,7 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
         tr = time.perf_counter()
         for handle in [h for h in running[k] if batch.steps_left(h) == 0]:
             req = running[k].pop(handle)
-            result = batch.read(handle, *fields)
+            result = batch.read(handle, *(fields[req.kind] if isinstance(fields, dict) else fields))
             batch.remove(handle)
             t_done = time.perf_counter() - t0
             done[req.index] = {"latency": t_done - (req.arrival if realtime else 0.0), "finished": t_done}

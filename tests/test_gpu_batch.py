@@ -122,6 +122,33 @@ def test_batch_dense_packing_matches_alone(compensated):
         check_same(k, got[k], want[k])
 
 
+def test_batch_fluid_static_body_matches_alone_and_oracle():
+    """A fluid scene whose lowest layer is a static body (body_number 1,
+    body_dynamic [0]): the body numbers move with the particles through
+    every reorder, so the same particles stay frozen in the batch as alone
+    (kernels/common.cl:699-704) and as in the FP32 oracle."""
+    from gpu_util import oracle_state
+    from sph.batch import Batch
+    sc = {"fS": scenes.dam_break(10, 9, 7, 0.02, 0.04, 1e-4, seed=4, steps=8, static_floor=True, name="fS"),
+          "cA": scenes.cantilever(12, 4, 4, steps=8, name="cA")}
+    say("batch fluid with a static floor")
+    want = {k: alone(s, 1) for k, s in sc.items()}
+    st = oracle_state(sc["fS"], params={"compensated": 1})
+    st.step("fluid", sc["fS"].steps)
+    floor = sc["fS"].fields["body_number"] == 1
+    assert np.array_equal(st.f["position"][floor], sc["fS"].fields["position"][floor])
+    assert compare("fS position alone vs oracle", want["fS"]["position"], st.f["position"]) == 0
+    batch = Batch(slots=1 << 14, cells=1 << 16, scenes=8, compensated_sums=1, graph_steps=4)
+    try:
+        h = {k: batch.add(s) for k, s in sc.items()}
+        batch.step(8)
+        got = {k: batch.read(h[k], *FIELDS[s.system]) for k, s in sc.items()}
+    finally:
+        batch.free()
+    for k in sc:
+        check_same(k, got[k], want[k])
+
+
 def test_batch_admission_when_full():
     """A scene that does not fit is refused (None) and fits after removals."""
     from sph.batch import Batch
