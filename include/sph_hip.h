@@ -208,6 +208,21 @@ int get_kernel_stats_hip(psdata_hip *pso, int capacity, const char **names,
  * (3), total kinetic energy, and the centre of mass (3).  out has 7 doubles. */
 int global_sums_hip(psdata_hip *pso, double *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the reductions of global_sums_hip over caller-owned
+ * device arrays, allocating nothing.  position: float[n][3], velocity:
+ * float[n][3] or NULL (zero velocities), mass: the particle mass, out7: 7
+ * doubles of device memory (momentum, kinetic energy, centre of mass),
+ * stream: a hipStream_t or NULL (asynchronous on that stream).  The partial
+ * sums live in one library-resident array of 7 x 256 doubles (14 KiB) per
+ * GPU, held until the process ends; every entry is rewritten by each call,
+ * and calls that overlap on one GPU race for it.  n is at most 1431655765
+ * (larger n is an error).  Equal inputs give bitwise the output of
+ * global_sums_hip over a handle whose slots hold them in the same order. */
+int global_sums_noalloc_hip(const float *position, const float *velocity, unsigned n, float mass, double *out7,
+                            void *stream);
+
 /* No counterpart in the reference (its work-group size is fixed at 256,
  * src/opencl/particle_system_host.c:361-528): threads per block (64, 128 or
  * 256) of the neighbour passes that accept any of them, process-wide;

@@ -27,6 +27,7 @@ SOURCES = [
     "kernels_bodies.hip",
     "runtime.hip",
     "batch.hip",
+    "sums_noalloc.hip",
 ]
 
 ARCH = os.environ.get("SPH_HIP_ARCH", "gfx950")

@@ -856,6 +856,7 @@ extern "C" int init_hip(void) {
     if (const char *env = std::getenv("SPH_HIP_DEVICE")) dev = std::atoi(env);
     if (dev < 0 || dev >= count) return fail("SPH_HIP_DEVICE=%d out of range (%d devices)", dev, count);
     HIP_TRY(hipSetDevice(dev));
+    if (sphhip::noalloc_bind_current_device(nullptr)) return -1;
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, dev));
     note(1, "Chosen device: %s (%s, %d CUs)\n", prop.name, prop.gcnArchName, prop.multiProcessorCount);

@@ -39,4 +39,8 @@ int current_device(int *device);
 int params_from_blob(const psdata &meta, const char *blob, const KernelSet &k, unsigned n, bool has_grid,
                      int compensated, int polar_fp64, sphk::StepParams *out);
 
+// Device address of the allocation-free entry points' scratch array on the
+// current device (looked up once per device); *out may be NULL.
+int noalloc_bind_current_device(double **out);
+
 }  // namespace sphhip
