@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <map>
 #include <string>
@@ -64,7 +65,7 @@ __device__ __forceinline__ unsigned linear_slot() { return blockIdx.x * blockDim
 
 // Slot i holds a particle of a resident scene (finished or not).
 __device__ __forceinline__ bool live(const SceneTable &ps, unsigned i) {
-    if (i >= ps.t.capacity) return false;
+    if (i >= ps.capacity()) return false;
     const unsigned s = ps.t.scene_of[i];
     if (s == kNoScene) return false;
     const StepParams &p = ps.t.scenes[s];
@@ -78,14 +79,14 @@ __device__ __forceinline__ bool binned(const SceneTable &ps, unsigned i) {
 
 // Cell key and arrival rank of every binned slot; threads k < nsegs also
 // set the padding cell of segment k to the number of its slots that are not
-// binned.
+// binned.  The grid covers at least max(used slots, segments).
 __global__ void __launch_bounds__(kBlock) k_batch_key_count(SceneTable ps, const float *__restrict__ pos,
-                                                            const Segment *__restrict__ segs, unsigned nsegs,
+                                                            const Segment *__restrict__ segs,
                                                             unsigned *__restrict__ key,
                                                             unsigned *__restrict__ local,
                                                             unsigned *__restrict__ count) {
     const unsigned s = linear_slot();
-    if (s < nsegs) {
+    if (s < ps.t.layout->nsegs) {
         const Segment g = segs[s];
         const unsigned used =
             g.scene != kNoScene && ps.t.steps_left[g.scene] > 0u ? ps.t.scenes[g.scene].n : 0u;
@@ -222,14 +223,15 @@ struct sph_batch {
     StepParams *d_scenes = nullptr;
     unsigned *d_steps_left = nullptr, *body_dynamic = nullptr;
     Segment *d_segs = nullptr;
+    sphk::BatchLayout *d_layout = nullptr;   // what the passes read of the layout
 
     // per-system chunk maps (fluid, corotational, svk): the 64-slot chunks of
     // the running scenes of each, rebuilt on layout changes
     static constexpr int kMaps = 3;
     unsigned *d_chunk_slot[kMaps] = {}, *d_chunk_end[kMaps] = {};
-    unsigned nchunks[kMaps] = {};
     size_t max_chunks = 0;
-    unsigned long long maps_version = 0;
+    unsigned long long uploaded_version = 0;   // layout version on the device
+    sphk::BatchLayout layout{};                // as uploaded
 
     // host mirror
     std::vector<StepParams> scenes;
@@ -238,8 +240,7 @@ struct sph_batch {
     std::vector<Segment> segs;             // in slot (and cell) order
     unsigned used_slots = 0, used_cells = 0;
     unsigned setup_pending = 0;
-    bool layout_dirty = true;              // segs / used extents changed since the last upload
-    unsigned long long layout_version = 1;
+    unsigned long long layout_version = 1;   // advances on every admission and removal
 
     // per-pass timing
     bool profiling = false;
@@ -248,12 +249,12 @@ struct sph_batch {
     std::vector<hipEvent_t> event_pool;
     std::vector<std::string> stat_names;
 
-    // graph replay of K steps
+    // graph replay of K steps, one captured sequence per state-buffer parity,
+    // systems present, set-up step and size class of the work (the passes
+    // read the layout's counts from the device, so a sequence replays for
+    // every layout of its class)
     unsigned graph_steps = 0;
-    hipGraphExec_t graph = nullptr;
-    unsigned long long graph_version = 0;
-    int graph_cur = 0;                     // state buffer parity the graph starts from
-    unsigned long long eager_version = 0;  // layout last stepped without a graph
+    std::map<std::array<unsigned, 4>, hipGraphExec_t> graphs;
     unsigned long long graph_launches = 0, graph_captures = 0;
 };
 
@@ -287,8 +288,8 @@ size_t pool_bytes(size_t slots, size_t cells, size_t scenes) {
 
 void release(sph_batch *b) {
     if (!b) return;
-    if (b->stream) hipStreamSynchronize(b->stream);
-    if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->stream) hipStreamSynchronize(b->stream);   // no graph launch is still running
+    for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
     for (BatchTiming &t : b->pending) {
         hipEventDestroy(t.start);
         hipEventDestroy(t.stop);
@@ -303,24 +304,50 @@ void release(sph_batch *b) {
     delete b;
 }
 
+// Size class of a work count: launches cover the next power of two (at
+// least `floor`), so that a captured sequence serves every count of its class.
+unsigned size_class(unsigned units, unsigned floor, unsigned cap) {
+    if (units == 0) return 0;
+    unsigned c = floor;
+    while (c < units && c < (1u << 31)) c <<= 1;
+    return std::max(units, std::min(c, cap));
+}
+
+unsigned slot_class(const sph_batch *b) {
+    return size_class(std::max(b->layout.used_slots, b->layout.nsegs), 4096u,
+                      std::max(b->slot_cap, 2 * b->scene_cap + 1));
+}
+
 sphk::BatchTable table(const sph_batch *b) {
-    return sphk::BatchTable{b->d_scenes, b->scene_of, b->d_steps_left, b->used_slots, nullptr, nullptr, 0};
+    return sphk::BatchTable{b->d_scenes, b->scene_of, b->d_steps_left, b->d_layout, nullptr, nullptr, 0,
+                            slot_class(b)};
 }
 
 enum MapIndex { kMapFluid = 0, kMapCorot = 1, kMapSvk = 2 };
+
+// Chunks every system's passes cover: the size class of the longest map
+// (one class for the three, so that few sequences serve every mix).
+unsigned chunk_class(const sph_batch *b) {
+    const unsigned most = std::max(b->layout.nchunks[0], std::max(b->layout.nchunks[1], b->layout.nchunks[2]));
+    return size_class(most, 64u, static_cast<unsigned>(b->max_chunks));
+}
 
 // The scene table restricted to the chunks of one system's running scenes.
 sphk::BatchTable table(const sph_batch *b, int map) {
     sphk::BatchTable t = table(b);
     t.chunk_slot = b->d_chunk_slot[map];
     t.chunk_end = b->d_chunk_end[map];
-    t.nchunks = b->nchunks[map];
+    t.map = static_cast<unsigned>(map);
+    t.grid_units = b->layout.nchunks[map] ? chunk_class(b) : 0u;
     return t;
 }
 
-// Chunk maps of the scenes with steps left, in slot order.
-int build_chunk_maps(sph_batch *b) {
-    if (b->maps_version == b->layout_version) return 0;
+// Uploads what the passes read of the layout, when it changed: the
+// segments, the per-system chunk maps (the 64-slot chunks of the scenes with
+// steps left, in slot order) and the counts.  Enqueued on the batch stream,
+// so that it takes effect after the steps enqueued before it.
+int upload_layout(sph_batch *b) {
+    if (b->uploaded_version == b->layout_version) return 0;
     std::vector<unsigned> slot[sph_batch::kMaps], end[sph_batch::kMaps];
     for (const Segment &g : b->segs) {
         if (g.scene == kNoScene || b->steps_left[g.scene] == 0) continue;
@@ -331,17 +358,26 @@ int build_chunk_maps(sph_batch *b) {
             end[m].push_back(p.p0 + p.n);
         }
     }
+    sphk::BatchLayout lay{};
+    lay.used_slots = b->used_slots;
+    lay.used_cells = b->used_cells;
+    lay.nsegs = static_cast<unsigned>(b->segs.size());
     for (int m = 0; m < sph_batch::kMaps; ++m) {
         if (slot[m].size() > b->max_chunks) return fail("internal: chunk map overflow");
-        b->nchunks[m] = static_cast<unsigned>(slot[m].size());
+        lay.nchunks[m] = static_cast<unsigned>(slot[m].size());
         if (slot[m].empty()) continue;
         SPH_HIP_TRY(hipMemcpyAsync(b->d_chunk_slot[m], slot[m].data(), slot[m].size() * 4, hipMemcpyHostToDevice,
                                    b->stream));
         SPH_HIP_TRY(hipMemcpyAsync(b->d_chunk_end[m], end[m].data(), end[m].size() * 4, hipMemcpyHostToDevice,
                                    b->stream));
     }
-    SPH_HIP_TRY(hipStreamSynchronize(b->stream));   // the host vectors go out of scope
-    b->maps_version = b->layout_version;
+    if (!b->segs.empty())
+        SPH_HIP_TRY(hipMemcpyAsync(b->d_segs, b->segs.data(), b->segs.size() * sizeof(Segment), hipMemcpyHostToDevice,
+                                   b->stream));
+    SPH_HIP_TRY(hipMemcpyAsync(b->d_layout, &lay, sizeof(lay), hipMemcpyHostToDevice, b->stream));
+    SPH_HIP_TRY(hipStreamSynchronize(b->stream));   // the host copies go out of scope
+    b->layout = lay;
+    b->uploaded_version = b->layout_version;
     return 0;
 }
 
@@ -364,18 +400,6 @@ sphk::FieldPtrs field_ptrs(const sph_batch *b) {
     f.celloffset = b->celloffset;
     f.gridcount = b->gridcount;
     return f;
-}
-
-int upload_layout(sph_batch *b) {
-    if (!b->layout_dirty) return 0;
-    if (!b->segs.empty())
-    {
-        SPH_HIP_TRY(hipMemcpyAsync(b->d_segs, b->segs.data(), b->segs.size() * sizeof(Segment),
-                                   hipMemcpyHostToDevice, b->stream));
-        SPH_HIP_TRY(hipStreamSynchronize(b->stream));
-    }
-    b->layout_dirty = false;
-    return 0;
 }
 
 int upload_scene(sph_batch *b, unsigned s) {
@@ -448,17 +472,18 @@ int enqueue_bins(sph_batch *b) {
     const sphk::SceneTable all{t, sphk::kAllScenes};
     const SlotState a = b->st[b->cur];
     const SlotState n = b->st[b->cur ^ 1];
-    const dim3 grid = sphk::blocks(b->used_slots), block(sphk::kBlock);
-    const unsigned nsegs = static_cast<unsigned>(b->segs.size());
+    const dim3 grid = all.launch_blocks(sphk::kBlock, false), block(sphk::kBlock);
+    const unsigned *used_cells = &b->d_layout->used_cells;
     if (timed(b, "batch_key_count", [&] {
-            hipError_t e = sphk::launch_fill_u32(b->gridcount, b->used_cells, 0u, s);
+            hipError_t e = sphk::launch_fill_u32_dev(b->gridcount, used_cells, 0u, s);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(sphk::k_batch_key_count, sphk::blocks(std::max<size_t>(b->used_slots, nsegs)), block,
-                               0, s, all, a.pos, b->d_segs, nsegs, b->bins.key, b->bins.local, b->gridcount);
+            hipLaunchKernelGGL(sphk::k_batch_key_count, grid, block, 0, s, all, a.pos, b->d_segs, b->bins.key,
+                               b->bins.local, b->gridcount);
             return hipGetLastError();
         }) ||
         timed(b, "batch_scan", [&] {
-            return sphk::launch_exclusive_scan(b->gridcount, b->used_cells, b->celloffset, b->bins.tile_sums, s);
+            return sphk::launch_exclusive_scan_dev(b->gridcount, used_cells, b->cell_cap, b->celloffset,
+                                                   b->bins.tile_sums, s);
         }) ||
         timed(b, "batch_scatter", [&] {
             hipLaunchKernelGGL(sphk::k_batch_scatter, grid, block, 0, s, all, b->bins.key, b->bins.local,
@@ -477,10 +502,11 @@ int enqueue_bins(sph_batch *b) {
 
 // One step of every scene with steps left.  `systems` selects the passes to
 // launch (bit SystemKind); `setup` adds the reference-configuration passes
-// for scenes flagged kSceneSetup.  The three systems' passes touch disjoint
-// slots, so unless per-pass timing is on they run on three streams, forked
-// after binning and joined before the integrator (parallel branches of a
-// captured graph).
+// for the scenes flagged kSceneSetup (none, in a replayed graph whose
+// scenes were all set up before) and clears the flags after them.  The three
+// systems' passes touch disjoint slots, so unless per-pass timing is on they
+// run on three streams, forked after binning and joined before the
+// integrator (parallel branches of a captured graph).
 int enqueue_step(sph_batch *b, unsigned systems, bool setup) {
     hipStream_t s = b->stream;
     const bool branch = !b->profiling;
@@ -498,8 +524,13 @@ int enqueue_step(sph_batch *b, unsigned systems, bool setup) {
         return -1;
     if (enqueue_bins(b)) return -1;
     const sphk::FieldPtrs f = field_ptrs(b);
-    if (setup && timed(b, "compute_original_density",
-                       [&] { return sphk::launch_batch_original_density(t, setup_sel, f, kahan, s); }))
+    if (setup && (timed(b, "compute_original_density",
+                        [&] { return sphk::launch_batch_original_density(t, setup_sel, f, kahan, s); }) ||
+                  timed(b, "batch_clear_setup", [&] {
+                      hipLaunchKernelGGL(sphk::k_batch_clear_setup, sphk::blocks(b->scene_cap), dim3(sphk::kBlock), 0,
+                                         s, b->scene_cap, b->d_scenes);
+                      return hipGetLastError();
+                  })))
         return -1;
     const bool has_c = (systems >> sphk::kCorotational) & 1u, has_v = (systems >> sphk::kSvk) & 1u;
     if (branch) {
@@ -558,7 +589,6 @@ void reindex(sph_batch *b) {
     for (unsigned s = 0; s < b->scene_cap; ++s) b->seg_of[s] = kNoScene;
     for (size_t j = 0; j < b->segs.size(); ++j)
         if (b->segs[j].scene != kNoScene) b->seg_of[b->segs[j].scene] = static_cast<unsigned>(j);
-    b->layout_dirty = true;
     b->layout_version += 1;
 }
 
@@ -725,7 +755,8 @@ extern "C" int sph_batch_create(unsigned slots, unsigned cells, unsigned scenes,
          dev_alloc(b, &b->gridcount, C) || dev_alloc(b, &b->celloffset, C) ||
          dev_alloc(b, &b->bins.tile_sums, tiles) || dev_alloc(b, &b->d_scenes, scenes) ||
          dev_alloc(b, &b->d_steps_left, scenes) || dev_alloc(b, &b->body_dynamic, sphk::kBodiesPerScene * scenes) ||
-         dev_alloc(b, &b->d_segs, 2 * static_cast<size_t>(scenes) + 1);
+         dev_alloc(b, &b->d_segs, 2 * static_cast<size_t>(scenes) + 1) ||
+         dev_alloc(b, &b->d_layout, 1);
     b->max_chunks = S / sphk::kChunk + scenes;
     for (int m = 0; m < sph_batch::kMaps && !rc; ++m)
         rc = dev_alloc(b, &b->d_chunk_slot[m], b->max_chunks) || dev_alloc(b, &b->d_chunk_end[m], b->max_chunks);
@@ -871,68 +902,104 @@ extern "C" int sph_batch_remove(sph_batch *b, int scene) {
     return 0;
 }
 
+namespace {
+
+void destroy_graphs(sph_batch *b) {
+    for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+}
+
+}  // namespace
+
 extern "C" int sph_batch_set_graph_steps(sph_batch *b, unsigned steps_per_graph) {
     if (!b) return fail("null batch");
     if (steps_per_graph % 2u != 0u) return fail("steps per graph must be even (the state buffers alternate)");
+    if (steps_per_graph == b->graph_steps) return 0;
+    SPH_HIP_TRY(hipStreamSynchronize(b->stream));   // no launch of the old graphs is still running
+    destroy_graphs(b);
     b->graph_steps = steps_per_graph;
     return 0;
 }
 
+namespace {
+
+constexpr size_t kMaxGraphs = 256;   // cached sequences per batch
+
+// Captures graph_steps batch steps from the current buffer parity for the
+// given systems, with the set-up passes in the first step when `setup` (they
+// act on the scenes flagged kSceneSetup, if any).
+int capture_graph(sph_batch *b, unsigned systems, bool setup, hipGraphExec_t *out) {
+    const int cur0 = b->cur;
+    SPH_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (unsigned k = 0; k < b->graph_steps && !rc; ++k) rc = enqueue_step(b, systems, setup && k == 0);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    b->cur = cur0;   // an even number of steps: the parity the graph starts from
+    if (rc || e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return rc ? -1 : fail("graph capture failed: %s", hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        *out = nullptr;
+        return fail("hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    }
+    b->graph_captures += 1;
+    return 0;
+}
+
+// The cached sequence for the current parity, systems and size classes,
+// captured on first use.  With solids present its first step carries the
+// set-up passes, which cost a flag test per slot when no scene is pending.
+int graph_for(sph_batch *b, unsigned systems, hipGraphExec_t *out) {
+    const unsigned solids = (1u << sphk::kCorotational) | (1u << sphk::kSvk);
+    const bool setup = (systems & solids) != 0;
+    const std::array<unsigned, 4> key = {static_cast<unsigned>(b->cur), systems, slot_class(b), chunk_class(b)};
+    auto it = b->graphs.find(key);
+    if (it != b->graphs.end()) {
+        *out = it->second;
+        return 0;
+    }
+    if (b->graphs.size() >= kMaxGraphs) {
+        SPH_HIP_TRY(hipStreamSynchronize(b->stream));   // none of them is still running
+        destroy_graphs(b);
+    }
+    hipGraphExec_t g = nullptr;
+    if (capture_graph(b, systems, setup, &g)) return -1;
+    b->graphs[key] = g;
+    *out = g;
+    return 0;
+}
+
+void clear_setup_mirror(sph_batch *b) {
+    for (unsigned s = 0; s < b->scene_cap; ++s) b->scenes[s].flags &= ~sphk::kSceneSetup;
+    b->setup_pending = 0;
+}
+
+}  // namespace
+
 extern "C" int sph_batch_step(sph_batch *b, unsigned num_steps) {
     if (!b) return fail("null batch");
     if (hipSetDevice(b->device) != hipSuccess) return fail("hipSetDevice failed");
-    if (upload_layout(b) || build_chunk_maps(b)) return -1;
+    if (upload_layout(b)) return -1;
     unsigned done = 0;
     while (done < num_steps) {
         if (b->used_slots == 0) break;
-        if (b->setup_pending) {
-            if (enqueue_step(b, systems_present(b), true)) return -1;
-            for (unsigned s = 0; s < b->scene_cap; ++s) b->scenes[s].flags &= ~sphk::kSceneSetup;
-            hipLaunchKernelGGL(sphk::k_batch_clear_setup, sphk::blocks(b->scene_cap), dim3(sphk::kBlock), 0,
-                               b->stream, b->scene_cap, b->d_scenes);
-            SPH_HIP_TRY(hipGetLastError());
-            b->setup_pending = 0;
-            done += 1;
-        } else if (b->graph_steps > 0 && !b->profiling && num_steps - done >= b->graph_steps &&
-                   (b->graph_version == b->layout_version || b->eager_version == b->layout_version)) {
-            // A layout is captured once it has run one round of steps without
-            // a graph: layouts that change every round are not worth a capture.
-            if (!b->graph || b->graph_version != b->layout_version || b->graph_cur != b->cur) {
-                if (b->graph) {
-                    SPH_HIP_TRY(hipGraphExecDestroy(b->graph));
-                    b->graph = nullptr;
-                }
-                const int cur0 = b->cur;
-                SPH_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                int rc = 0;
-                const unsigned all = (1u << sphk::kFluid) | (1u << sphk::kCorotational) | (1u << sphk::kSvk);
-                for (unsigned k = 0; k < b->graph_steps && !rc; ++k) rc = enqueue_step(b, all, false);
-                hipGraph_t g = nullptr;
-                const hipError_t e = hipStreamEndCapture(b->stream, &g);
-                b->cur = cur0;
-                if (rc || e != hipSuccess) {
-                    if (g) hipGraphDestroy(g);
-                    return rc ? -1 : fail("graph capture failed: %s", hipGetErrorString(e));
-                }
-                const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-                hipGraphDestroy(g);
-                if (ei != hipSuccess) {
-                    b->graph = nullptr;
-                    return fail("hipGraphInstantiate failed: %s", hipGetErrorString(ei));
-                }
-                b->graph_version = b->layout_version;
-                b->graph_cur = cur0;
-                b->graph_captures += 1;
-            }
-            SPH_HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+        const bool setup = b->setup_pending > 0;
+        const unsigned systems = systems_present(b);
+        if (b->graph_steps > 0 && !b->profiling && num_steps - done >= b->graph_steps) {
+            hipGraphExec_t g = nullptr;
+            if (graph_for(b, systems, &g)) return -1;
+            SPH_HIP_TRY(hipGraphLaunch(g, b->stream));
             b->graph_launches += 1;
             done += b->graph_steps;
-            // the graph's steps leave the buffer parity where it started
         } else {
-            if (enqueue_step(b, systems_present(b), false)) return -1;
+            if (enqueue_step(b, systems, setup)) return -1;
             done += 1;
-            if (done >= b->graph_steps) b->eager_version = b->layout_version;
         }
+        if (setup) clear_setup_mirror(b);
     }
     // mirror of the device step counters for the steps just enqueued
     for (unsigned s = 0; s < b->scene_cap; ++s)

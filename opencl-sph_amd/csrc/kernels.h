@@ -105,6 +105,11 @@ struct BinScratch {
 constexpr unsigned kScanTile = 4096;
 
 hipError_t launch_fill_u32(unsigned *p, size_t count, unsigned value, hipStream_t s);
+// The same with the count read from device memory (*count, at most
+// max_count), as fixed grids: launches whose size does not depend on it.
+hipError_t launch_fill_u32_dev(unsigned *p, const unsigned *count, unsigned value, hipStream_t s);
+hipError_t launch_exclusive_scan_dev(const unsigned *in, const unsigned *count, unsigned max_count, unsigned *out,
+                                     unsigned *tile_sums, hipStream_t s);
 hipError_t launch_iota_u32(unsigned *p, size_t count, hipStream_t s);
 hipError_t launch_cell_keys(const float *position, unsigned n, const GridGeom &g, unsigned *key_out,
                             hipStream_t s);
@@ -165,14 +170,27 @@ constexpr unsigned kNoScene = 0xffffffffu;
 // so each wavefront works on one scene and a pass for one system launches
 // only over that system's scenes, spread evenly over the XCDs.
 constexpr unsigned kChunk = 64;
+// The part of a pool's layout that changes with admissions and removals.  It
+// lives in device memory and every batch pass reads its counts there:
+// threads past a count have no work, so a launch sized for at least the
+// current work is correct for every layout, and a step sequence captured with
+// grids of a given size class replays unchanged for every layout of that
+// class.
+struct BatchLayout {
+    unsigned used_slots;       // slots [0, used_slots) are tiled by segments
+    unsigned used_cells;       // cells [0, used_cells) likewise
+    unsigned nsegs;            // segments
+    unsigned nchunks[3];       // chunk map lengths (fluid, corotational, svk)
+};
 struct BatchTable {
     const StepParams *scenes;
     const unsigned *scene_of;
     const unsigned *steps_left;   // per scene
-    unsigned capacity;            // the pool's used slots
-    const unsigned *chunk_slot;   // nullptr: no chunk map
+    const BatchLayout *layout;    // device
+    const unsigned *chunk_slot;   // nullptr: no chunk map (every used slot)
     const unsigned *chunk_end;
-    unsigned nchunks;
+    unsigned map;                 // chunk map index into layout->nchunks
+    unsigned grid_units;          // host: slots (no chunk map) or chunks the launch covers
 };
 
 hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipStream_t s);

@@ -599,9 +599,9 @@ inline dim3 nbr_grid(unsigned n, unsigned block) { return dim3(xcd_grid((n + blo
 // ---- launchers shared by both parameter sources --------------------------
 template <class PS>
 hipError_t density(const PS &ps, const FieldPtrs &f, bool original, int compensated, hipStream_t s) {
-    if (ps.slots() == 0) return hipSuccess;
+    if (!ps.has_work()) return hipSuccess;
     const unsigned nb = neighbour_block();
-    const dim3 g = nbr_grid(ps.slots(), nb), b(nb);
+    const dim3 g = ps.launch_blocks(nb, true), b(nb);
     const bool k = (compensated & kKahanDensity) != 0;
     if (original) {
         if (k) hipLaunchKernelGGL((k_density<PS, true, true>), g, b, 0, s, ps, f);
@@ -615,8 +615,8 @@ hipError_t density(const PS &ps, const FieldPtrs &f, bool original, int compensa
 
 template <class PS>
 hipError_t density_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated, hipStream_t s) {
-    if (ps.slots() == 0) return hipSuccess;
-    const dim3 g = nbr_grid(ps.slots(), kListBlock), b(kListBlock);
+    if (!ps.has_work()) return hipSuccess;
+    const dim3 g = ps.launch_blocks(kListBlock, true), b(kListBlock);
     if (compensated & kKahanDensity) hipLaunchKernelGGL((k_density_listed<PS, true>), g, b, 0, s, ps, f, c);
     else hipLaunchKernelGGL((k_density_listed<PS, false>), g, b, 0, s, ps, f, c);
     return hipGetLastError();
@@ -625,8 +625,8 @@ hipError_t density_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, 
 template <class PS>
 hipError_t density_solid_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated,
                                 hipStream_t s) {
-    if (ps.slots() == 0) return hipSuccess;
-    const dim3 g = nbr_grid(ps.slots(), kListBlock), b(kListBlock);
+    if (!ps.has_work()) return hipSuccess;
+    const dim3 g = ps.launch_blocks(kListBlock, true), b(kListBlock);
     if (compensated & kKahanDensity) hipLaunchKernelGGL((k_density_solid_listed<PS, true>), g, b, 0, s, ps, f, c);
     else hipLaunchKernelGGL((k_density_solid_listed<PS, false>), g, b, 0, s, ps, f, c);
     return hipGetLastError();
@@ -634,9 +634,9 @@ hipError_t density_solid_listed(const PS &ps, const FieldPtrs &f, const PairCach
 
 template <class PS>
 hipError_t forces_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated, hipStream_t s) {
-    if (ps.slots() == 0) return hipSuccess;
+    if (!ps.has_work()) return hipSuccess;
     const unsigned nb = neighbour_block();
-    const dim3 g = nbr_grid(ps.slots(), nb), b(nb);
+    const dim3 g = ps.launch_blocks(nb, true), b(nb);
     const bool kahan = (compensated & kKahanForces) != 0;
     switch (fluid_pair_group()) {
     case 12:
@@ -656,15 +656,15 @@ hipError_t forces_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, i
 
 template <class PS>
 hipError_t step_forward(const PS &ps, const FieldPtrs &f, hipStream_t s) {
-    if (ps.slots() == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_step_forward<PS>), grid_for(ps.slots()), dim3(kBlock), 0, s, ps, f);
+    if (!ps.has_work()) return hipSuccess;
+    hipLaunchKernelGGL((k_step_forward<PS>), ps.launch_blocks(kBlock, false), dim3(kBlock), 0, s, ps, f);
     return hipGetLastError();
 }
 
 template <class PS>
 hipError_t init_original_position(const PS &ps, const FieldPtrs &f, hipStream_t s) {
-    if (ps.slots() == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_init_original_position<PS>), grid_for(ps.slots()), dim3(kBlock), 0, s, ps, f);
+    if (!ps.has_work()) return hipSuccess;
+    hipLaunchKernelGGL((k_init_original_position<PS>), ps.launch_blocks(kBlock, false), dim3(kBlock), 0, s, ps, f);
     return hipGetLastError();
 }
 

@@ -10,6 +10,8 @@
 // is what a sequential run of the reference produces, so gridcell, gridcount,
 // celloffset and cellparticles are deterministic and bit-exact against the
 // sequential oracle.
+#include <algorithm>
+
 #include "cells.h"
 
 #pragma clang fp contract(off)
@@ -18,11 +20,18 @@ namespace sphk {
 namespace {
 
 constexpr unsigned kBlock = 256;
+constexpr unsigned kDevGrid = 2048;   // blocks of the launches whose count is read from the device
 
 __global__ void k_fill_u32(unsigned *p, size_t count, unsigned value) {
     size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (; i < count; i += stride) p[i] = value;
+}
+
+// The count read from device memory (launches whose size must not depend on it).
+__global__ void k_fill_u32_dev(unsigned *p, const unsigned *count, unsigned value) {
+    const unsigned n = *count;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = value;
 }
 
 __global__ void k_iota_u32(unsigned *p, size_t count) {
@@ -80,9 +89,9 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned *t
 // scan of the tile sums, then each tile's own scan.  VEC: the arrays are
 // 16-byte aligned and full tiles move as 16-byte words.
 template <bool VEC>
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(const unsigned *__restrict__ in, unsigned count,
-                                                        unsigned *__restrict__ tile_sums) {
-    const unsigned base = blockIdx.x * kScanTile;
+__device__ __forceinline__ void scan_reduce_tile(const unsigned *__restrict__ in, unsigned count, unsigned tile,
+                                                 unsigned *__restrict__ tile_sums) {
+    const unsigned base = tile * kScanTile;
     unsigned sum = 0;
     if (VEC && base + kScanTile <= count) {
         const uint4 *v = reinterpret_cast<const uint4 *>(in + base);
@@ -99,10 +108,16 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(const unsigned *__restri
     }
     unsigned total;
     block_exclusive_scan(sum, &total);
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+    if (threadIdx.x == 0) tile_sums[tile] = total;
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_tile_sums(unsigned *tile_sums, unsigned ntiles) {
+template <bool VEC>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const unsigned *__restrict__ in, unsigned count,
+                                                        unsigned *__restrict__ tile_sums) {
+    scan_reduce_tile<VEC>(in, count, blockIdx.x, tile_sums);
+}
+
+__device__ __forceinline__ void scan_tile_sums(unsigned *tile_sums, unsigned ntiles) {
     unsigned carry = 0;
     for (unsigned base = 0; base < ntiles; base += kBlock) {
         unsigned i = base + threadIdx.x;
@@ -114,6 +129,10 @@ __global__ void __launch_bounds__(kBlock) k_scan_tile_sums(unsigned *tile_sums, 
     }
 }
 
+__global__ void __launch_bounds__(kBlock) k_scan_tile_sums(unsigned *tile_sums, unsigned ntiles) {
+    scan_tile_sums(tile_sums, ntiles);
+}
+
 // Tiles up to which each tile of the final pass sums the tile sums before it
 // itself (at most kPrefixTiles values per block) instead of a separate
 // one-block scan of the tile sums.
@@ -122,18 +141,17 @@ constexpr unsigned kPrefixTiles = 4096;
 // PREFIX: tile_sums holds the raw tile sums and each block adds up those of
 // the tiles before it; otherwise tile_sums holds their exclusive scan.
 template <bool VEC, bool PREFIX>
-__global__ void __launch_bounds__(kBlock) k_scan_final(const unsigned *__restrict__ in, unsigned count,
-                                                       const unsigned *__restrict__ tile_sums,
-                                                       unsigned *__restrict__ out) {
+__device__ __forceinline__ void scan_final_tile(const unsigned *__restrict__ in, unsigned count, unsigned tile,
+                                                const unsigned *__restrict__ tile_sums, unsigned *__restrict__ out) {
     unsigned tile_off;
     if (PREFIX) {
         unsigned pre = 0;
-        for (unsigned k = threadIdx.x; k < blockIdx.x; k += kBlock) pre += tile_sums[k];
+        for (unsigned k = threadIdx.x; k < tile; k += kBlock) pre += tile_sums[k];
         block_exclusive_scan(pre, &tile_off);
     } else {
-        tile_off = tile_sums[blockIdx.x];
+        tile_off = tile_sums[tile];
     }
-    const unsigned base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    const unsigned base = tile * kScanTile + threadIdx.x * kScanItems;
     const bool full = VEC && base + kScanItems <= count;
     unsigned v[kScanItems];
     if (full) {
@@ -170,6 +188,41 @@ __global__ void __launch_bounds__(kBlock) k_scan_final(const unsigned *__restric
         for (unsigned k = 0; k < kScanItems; ++k)
             if (base + k < count) out[base + k] = o[k];
     }
+}
+
+template <bool VEC, bool PREFIX>
+__global__ void __launch_bounds__(kBlock) k_scan_final(const unsigned *__restrict__ in, unsigned count,
+                                                       const unsigned *__restrict__ tile_sums,
+                                                       unsigned *__restrict__ out) {
+    scan_final_tile<VEC, PREFIX>(in, count, blockIdx.x, tile_sums, out);
+}
+
+// Persistent forms: the count read from device memory, the blocks of a fixed
+// grid looping over the tiles (uniformly per block: the tile functions
+// synchronise the block).
+template <bool VEC>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce_dev(const unsigned *__restrict__ in,
+                                                            const unsigned *__restrict__ count_p,
+                                                            unsigned *__restrict__ tile_sums) {
+    const unsigned count = *count_p;
+    const unsigned ntiles = (count + kScanTile - 1) / kScanTile;
+    for (unsigned tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+        scan_reduce_tile<VEC>(in, count, tile, tile_sums);
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_tile_sums_dev(unsigned *tile_sums, const unsigned *count_p) {
+    scan_tile_sums(tile_sums, (*count_p + kScanTile - 1) / kScanTile);
+}
+
+template <bool VEC>
+__global__ void __launch_bounds__(kBlock) k_scan_final_dev(const unsigned *__restrict__ in,
+                                                           const unsigned *__restrict__ count_p,
+                                                           const unsigned *__restrict__ tile_sums,
+                                                           unsigned *__restrict__ out) {
+    const unsigned count = *count_p;
+    const unsigned ntiles = (count + kScanTile - 1) / kScanTile;
+    for (unsigned tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+        scan_final_tile<VEC, false>(in, count, tile, tile_sums, out);
 }
 
 __global__ void k_scatter_unsorted(unsigned n, const unsigned *__restrict__ key,
@@ -270,6 +323,11 @@ hipError_t launch_fill_u32(unsigned *p, size_t count, unsigned value, hipStream_
     return hipGetLastError();
 }
 
+hipError_t launch_fill_u32_dev(unsigned *p, const unsigned *count, unsigned value, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32_dev, dim3(kDevGrid), dim3(kBlock), 0, s, p, count, value);
+    return hipGetLastError();
+}
+
 hipError_t launch_iota_u32(unsigned *p, size_t count, hipStream_t s) {
     if (count == 0) return hipSuccess;
     unsigned blocks = blocks_for(count);
@@ -310,6 +368,19 @@ hipError_t launch_exclusive_scan(const unsigned *in, unsigned count, unsigned *o
         if (vec) hipLaunchKernelGGL((k_scan_final<true, false>), grid, block, 0, s, in, count, tile_sums, out);
         else hipLaunchKernelGGL((k_scan_final<false, false>), grid, block, 0, s, in, count, tile_sums, out);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_exclusive_scan_dev(const unsigned *in, const unsigned *count, unsigned max_count, unsigned *out,
+                                     unsigned *tile_sums, hipStream_t s) {
+    const unsigned max_tiles = (max_count + kScanTile - 1) / kScanTile;
+    const dim3 grid(std::max(1u, std::min(max_tiles, kDevGrid))), block(kBlock);
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0;
+    if (vec) hipLaunchKernelGGL(k_scan_reduce_dev<true>, grid, block, 0, s, in, count, tile_sums);
+    else hipLaunchKernelGGL(k_scan_reduce_dev<false>, grid, block, 0, s, in, count, tile_sums);
+    hipLaunchKernelGGL(k_scan_tile_sums_dev, dim3(1), block, 0, s, tile_sums, count);
+    if (vec) hipLaunchKernelGGL(k_scan_final_dev<true>, grid, block, 0, s, in, count, tile_sums, out);
+    else hipLaunchKernelGGL(k_scan_final_dev<false>, grid, block, 0, s, in, count, tile_sums, out);
     return hipGetLastError();
 }
 

@@ -504,16 +504,13 @@ __global__ void __launch_bounds__(kBlock) k_forces_solids(PS ps, FieldPtrs f, Pa
     }
 }
 
-inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
-inline dim3 nbr_grid(unsigned n, unsigned block) { return dim3(xcd_grid((n + block - 1) / block)); }
-
 // Launches kernel template K<MODE> as the list-only and overflow-scan pair
 // when `c` holds a solid list, else as one scan over every lane, with `nb`
 // threads per block.
 template <template <int> class K, class PS>
 hipError_t launch_modes(const PS &ps, const FieldPtrs &f, const PairCache &c, hipStream_t s, unsigned nb) {
-    if (ps.slots() == 0) return hipSuccess;
-    const dim3 g = nbr_grid(ps.slots(), nb), b(nb);
+    if (!ps.has_work()) return hipSuccess;
+    const dim3 g = ps.launch_blocks(nb, true), b(nb);
     if (c.valid == kSolidList) {
         hipLaunchKernelGGL(K<kListedOnly>::template kernel<PS>, g, b, 0, s, ps, f, c);
         hipLaunchKernelGGL(K<kScanOverflow>::template kernel<PS>, g, b, 0, s, ps, f, c);
@@ -565,8 +562,8 @@ hipError_t rotations_and_strains(const PS &ps, const FieldPtrs &f, const PairCac
 
 template <class PS>
 hipError_t stresses_corotational(const PS &ps, const FieldPtrs &f, hipStream_t s) {
-    if (ps.slots() == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_stresses_corot<PS>), grid_for(ps.slots()), dim3(kBlock), 0, s, ps, f);
+    if (!ps.has_work()) return hipSuccess;
+    hipLaunchKernelGGL((k_stresses_corot<PS>), ps.launch_blocks(kBlock, false), dim3(kBlock), 0, s, ps, f);
     return hipGetLastError();
 }
 
@@ -582,8 +579,8 @@ hipError_t deformations_svk(const PS &ps, const FieldPtrs &f, const PairCache &c
 
 template <class PS>
 hipError_t stresses_svk(const PS &ps, const FieldPtrs &f, hipStream_t s) {
-    if (ps.slots() == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_stresses_svk<PS>), grid_for(ps.slots()), dim3(kBlock), 0, s, ps, f);
+    if (!ps.has_work()) return hipSuccess;
+    hipLaunchKernelGGL((k_stresses_svk<PS>), ps.launch_blocks(kBlock, false), dim3(kBlock), 0, s, ps, f);
     return hipGetLastError();
 }
 

@@ -24,9 +24,11 @@ __device__ __forceinline__ unsigned xcd_block() {
     return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
 }
 
-// slot_xcd / slot_linear: the slot of this thread in a launch of
-// `block_threads`-thread blocks sized by slots(), with the XCD-aware or the
-// plain block order; inactive threads get a slot that active() rejects.
+// slot_xcd / slot_linear: the slot of this thread, with the XCD-aware or the
+// plain block order; threads without work get a slot that active() rejects.
+// launch_blocks: the grid of a pass (a multiple of 8 blocks for the XCD
+// order).  A single scene launches one thread per slot; a scene table covers
+// grid_units slots or chunks, at least what its layout holds.
 struct SingleScene {
     StepParams p;
     __device__ __forceinline__ bool active(unsigned i) const { return i < p.n; }
@@ -38,13 +40,19 @@ struct SingleScene {
     __device__ __forceinline__ unsigned slot_linear(unsigned block_threads) const {
         return blockIdx.x * block_threads + threadIdx.x;
     }
+    __host__ bool has_work() const { return p.n > 0; }
+    __host__ dim3 launch_blocks(unsigned block_threads, bool xcd) const {
+        const unsigned b = (p.n + block_threads - 1) / block_threads;
+        return dim3(xcd ? xcd_grid(b) : b);
+    }
 };
 
 struct SceneTable {
     BatchTable t;
     SceneFilter sel;
+    __device__ __forceinline__ unsigned capacity() const { return t.layout->used_slots; }
     __device__ __forceinline__ bool active(unsigned i) const {
-        if (i >= t.capacity) return false;
+        if (i >= capacity()) return false;
         const unsigned s = t.scene_of[i];
         if (s == kNoScene) return false;
         const StepParams &p = t.scenes[s];
@@ -52,21 +60,30 @@ struct SceneTable {
                (p.flags & sel.flag_mask) == sel.flag_mask;
     }
     __device__ __forceinline__ const StepParams &at(unsigned i) const { return t.scenes[t.scene_of[i]]; }
-    __host__ __device__ __forceinline__ unsigned slots() const {
-        return t.chunk_slot ? t.nchunks * kChunk : t.capacity;
-    }
-    __device__ __forceinline__ unsigned from_block(unsigned block, unsigned block_threads) const {
-        if (!t.chunk_slot) return block * block_threads + threadIdx.x;
-        const unsigned k = block * (block_threads / kChunk) + threadIdx.x / kChunk;
-        if (k >= t.nchunks) return kNoScene;
+    // Chunk k of the map: slots [chunk_slot[k], min(chunk_slot[k] + 64, chunk_end[k])).
+    __device__ __forceinline__ unsigned chunk_lane(unsigned k) const {
+        if (k >= t.layout->nchunks[t.map]) return kNoScene;
         const unsigned i = t.chunk_slot[k] + threadIdx.x % kChunk;
         return i < t.chunk_end[k] ? i : kNoScene;
     }
+    // XCD x takes the contiguous eighth [x * per, (x + 1) * per) of the
+    // chunks the layout holds, whatever the grid's size.
     __device__ __forceinline__ unsigned slot_xcd(unsigned block_threads) const {
-        return from_block(xcd_block(), block_threads);
+        if (!t.chunk_slot) return slot_linear(block_threads);
+        const unsigned cpb = block_threads / kChunk;
+        const unsigned per = (t.layout->nchunks[t.map] + kXcds - 1u) / kXcds;
+        const unsigned local = (blockIdx.x / kXcds) * cpb + threadIdx.x / kChunk;
+        return local < per ? chunk_lane((blockIdx.x % kXcds) * per + local) : kNoScene;
     }
     __device__ __forceinline__ unsigned slot_linear(unsigned block_threads) const {
-        return from_block(blockIdx.x, block_threads);
+        if (!t.chunk_slot) return blockIdx.x * block_threads + threadIdx.x;
+        return chunk_lane(blockIdx.x * (block_threads / kChunk) + threadIdx.x / kChunk);
+    }
+    __host__ bool has_work() const { return t.grid_units > 0; }
+    __host__ dim3 launch_blocks(unsigned block_threads, bool xcd) const {
+        const unsigned per_block = t.chunk_slot ? block_threads / kChunk : block_threads;
+        const unsigned b = (t.grid_units + per_block - 1) / per_block;
+        return dim3(xcd ? xcd_grid(b) : b);
     }
 };
 
