@@ -240,6 +240,61 @@ int sph_hip_get_fluid_pair_group(unsigned *entries);
  * of the autotuner's table). */
 int sph_hip_device_name(char *buf, unsigned len);
 
+/* ---- the reference's device API under its own names ----------------------
+ * src/opencl/particle_system_host.h:7-80 (and set_body_position/rotation,
+ * defined at src/opencl/particle_system_host.c:562-588): the same names,
+ * signatures and by-value device object, so that a caller written for the
+ * reference (scripts/sph/bindings.py:911-1033, scripts/sph/__init__.py,
+ * scripts/sph/systems.py) runs on this engine with only the library's path
+ * changed.  psdata_opencl has the reference's layout (the cl_mem, cl_kernel
+ * and cl_program members are pointers); the engine's handle travels in
+ * ps_prog, every OpenCL buffer member is NULL, kernel_names lists the kernel
+ * names call_for_all_particles accepts and num_grid_cells is the grid's cell
+ * count.  Like the reference, these return nothing and abort on an error
+ * (after printing sph_hip_last_error to stderr); the *_hip functions above
+ * return a status instead.  sync_psdata_host_to_device and
+ * sync_psdata_device_to_host move the whole blob, the *_fields_* variants
+ * the named fields (a missing name is skipped with a note), as the
+ * reference does. */
+typedef struct {
+    psdata host_psdata;              /* the psdata header as create_psdata_opencl received it */
+    unsigned int num_fields;
+    void *names, *names_offsets, *dimensions, *num_dimensions, *dimensions_offsets, *field_types, *entry_sizes;
+    void *data, *data_sizes, *data_offsets;
+    void *block_totals, *backup_prefix_sum;
+    size_t num_kernels;
+    const char *const *kernel_names;
+    const void *kernels;
+    void *ps_prog;                   /* psdata_hip * */
+    unsigned int num_grid_cells;
+    size_t po2_workgroup_size;
+    unsigned int num_blocks;
+} psdata_opencl;
+
+void init_opencl(void);                                                              /* :55 */
+psdata_opencl create_psdata_opencl(psdata *data, const char *file_list);            /* :56 */
+void assign_pso_kernel_args(psdata_opencl pso);       /* :57, no-op: arguments are bound per launch */
+void set_kernel_args_to_pso(psdata_opencl pso, void *kernel);                        /* :58, no-op */
+void free_psdata_opencl(psdata_opencl *pso);                                         /* :59 */
+void terminate_opencl(void);                                                         /* :60 */
+void sync_psdata_device_to_host(psdata data, psdata_opencl pso);                     /* :62 */
+void sync_psdata_fields_device_to_host(psdata data, psdata_opencl pso, size_t num_fields,
+                                       const char *const *field_names);              /* :63 */
+void sync_psdata_host_to_device(psdata data, psdata_opencl pso, int full);           /* :64 */
+void sync_psdata_fields_host_to_device(psdata data, psdata_opencl pso, size_t num_fields,
+                                       const char *const *field_names);              /* :65 */
+void populate_position_cuboid_device_opencl(psdata_opencl pso, float x1, float y1, float z1, float x2, float y2,
+                                            float z2, unsigned int xsize, unsigned int ysize,
+                                            unsigned int zsize);                     /* :67-72 */
+void rotate_particles_device_opencl(psdata_opencl pso, float angle_x, float angle_y, float angle_z); /* :73 */
+void call_for_all_particles_device_opencl(psdata_opencl pso, const char *kernel_name);             /* :74 */
+void compute_particle_bins_device_opencl(psdata_opencl pso);                                       /* :76 */
+void compute_density_device_opencl(psdata_opencl pso);                                             /* :78 */
+void compute_forces_device_opencl(psdata_opencl pso);                                              /* :79 */
+void step_forward_device_opencl(psdata_opencl pso);                                                /* :80 */
+void set_body_position_device_opencl(psdata_opencl pso, unsigned int body, float *position);      /* host.c:562 */
+void set_body_rotation_device_opencl(psdata_opencl pso, unsigned int body, float *euler_angles);  /* host.c:576 */
+
 /* ---- scene batch --------------------------------------------------------
  * No counterpart in the reference, which runs one psdata per OpenCL program
  * and steps it with one launch per kernel (src/opencl/particle_system_host.c:

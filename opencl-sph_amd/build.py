@@ -28,6 +28,7 @@ SOURCES = [
     "runtime.hip",
     "batch.hip",
     "sums_noalloc.hip",
+    "opencl_api.cpp",
 ]
 
 ARCH = os.environ.get("SPH_HIP_ARCH", "gfx950")

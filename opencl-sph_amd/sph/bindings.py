@@ -280,6 +280,65 @@ class PsdataHip:
         check(_set_body_rotation(self.handle, int(body), v))
 
 
+# ---- the reference's device names (scripts/sph/bindings.py:911-1062) -------
+# The by-value device object and the void, abort-on-error calls of
+# src/opencl/particle_system_host.h:7-80, so code written against the
+# reference's generated bindings runs unchanged.  The OpenCL members are
+# pointers (NULL here); ps_prog carries the engine's psdata_hip handle.
+class psdata_opencl(Structure):
+    _fields_ = [
+        ("host_psdata", psdata),
+        ("num_fields", c_uint),
+        ("names", c_void_p),
+        ("names_offsets", c_void_p),
+        ("dimensions", c_void_p),
+        ("num_dimensions", c_void_p),
+        ("dimensions_offsets", c_void_p),
+        ("field_types", c_void_p),
+        ("entry_sizes", c_void_p),
+        ("data", c_void_p),
+        ("data_sizes", c_void_p),
+        ("data_offsets", c_void_p),
+        ("block_totals", c_void_p),
+        ("backup_prefix_sum", c_void_p),
+        ("num_kernels", c_size_t),
+        ("kernel_names", POINTER(POINTER(c_char))),
+        ("kernels", c_void_p),
+        ("ps_prog", c_void_p),
+        ("num_grid_cells", c_uint),
+        ("po2_workgroup_size", c_size_t),
+        ("num_blocks", c_uint),
+    ]
+
+
+init_opencl = _fn("init_opencl", [], None)
+create_psdata_opencl = _fn("create_psdata_opencl", [POINTER(psdata), c_char_p], psdata_opencl)
+assign_pso_kernel_args = _fn("assign_pso_kernel_args", [psdata_opencl], None)
+set_kernel_args_to_pso = _fn("set_kernel_args_to_pso", [psdata_opencl, c_void_p], None)
+free_psdata_opencl = _fn("free_psdata_opencl", [POINTER(psdata_opencl)], None)
+terminate_opencl = _fn("terminate_opencl", [], None)
+sync_psdata_device_to_host = _fn("sync_psdata_device_to_host", [psdata, psdata_opencl], None)
+sync_psdata_fields_device_to_host = _fn("sync_psdata_fields_device_to_host",
+                                        [psdata, psdata_opencl, c_size_t, POINTER(c_char_p)], None)
+sync_psdata_host_to_device = _fn("sync_psdata_host_to_device", [psdata, psdata_opencl, c_int], None)
+sync_psdata_fields_host_to_device = _fn("sync_psdata_fields_host_to_device",
+                                        [psdata, psdata_opencl, c_size_t, POINTER(c_char_p)], None)
+populate_position_cuboid_device_opencl = _fn(
+    "populate_position_cuboid_device_opencl",
+    [psdata_opencl, c_float, c_float, c_float, c_float, c_float, c_float, c_uint, c_uint, c_uint], None)
+rotate_particles_device_opencl = _fn("rotate_particles_device_opencl", [psdata_opencl, c_float, c_float, c_float],
+                                     None)
+call_for_all_particles_device_opencl = _fn("call_for_all_particles_device_opencl", [psdata_opencl, c_char_p], None)
+compute_particle_bins_device_opencl = _fn("compute_particle_bins_device_opencl", [psdata_opencl], None)
+compute_density_device_opencl = _fn("compute_density_device_opencl", [psdata_opencl], None)
+compute_forces_device_opencl = _fn("compute_forces_device_opencl", [psdata_opencl], None)
+step_forward_device_opencl = _fn("step_forward_device_opencl", [psdata_opencl], None)
+set_body_position_device_opencl = _fn("set_body_position_device_opencl", [psdata_opencl, c_uint, POINTER(c_float)],
+                                      None)
+set_body_rotation_device_opencl = _fn("set_body_rotation_device_opencl", [psdata_opencl, c_uint, POINTER(c_float)],
+                                      None)
+
+
 # ---- scene batch ----------------------------------------------------------
 _batch_bytes = _fn("sph_batch_device_bytes", [c_uint, c_uint, c_uint], c_size_t)
 _batch_create = _fn("sph_batch_create", [c_uint, c_uint, c_uint, c_int, c_int, POINTER(c_void_p)], c_int)
