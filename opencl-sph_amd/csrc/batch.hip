@@ -276,7 +276,7 @@ size_t pool_bytes(size_t slots, size_t cells, size_t scenes) {
                             + (3 + 1 + 6 + 9) * 4                      // force, density, stress, deformation
                             + 2 * 4                                    // gridcell, scene_of
                             + 5 * 4                                    // bin scratch
-                            + (sphk::kListCap + 2) * 4                 // pair cache
+                            + (sphk::kListCap + 1 + 4 * sphk::kFluidPlanes) * 4   // pair cache, fluid records
                             + 9 * 4                                    // readback staging
                             + 1;                                       // chunk maps (3 x 8 B per 64 slots)
     const size_t per_cell = 2 * 4;
@@ -745,13 +745,13 @@ extern "C" int sph_batch_create(unsigned slots, unsigned cells, unsigned scenes,
              dev_alloc(b, &st.body, S) || dev_alloc(b, &st.id, S);
     }
     unsigned *pair_list = nullptr, *pair_count = nullptr;
-    float *pair_pressure = nullptr;
+    float4 *pair_rec = nullptr;
     rc = rc || dev_alloc(b, &b->force, 3 * S) || dev_alloc(b, &b->density, S) || dev_alloc(b, &b->stress, 6 * S) ||
          dev_alloc(b, &b->deformation, 9 * S) || dev_alloc(b, &b->gridcell, S) || dev_alloc(b, &b->scene_of, S) ||
          dev_alloc(b, &b->bins.key, S) || dev_alloc(b, &b->bins.local, S) ||
          dev_alloc(b, &b->bins.tmp_id, S) || dev_alloc(b, &b->bins.tmp_src, S) || dev_alloc(b, &b->bins.tmp_key, S) ||
          dev_alloc(b, &pair_list, sphk::kListCap * S) || dev_alloc(b, &pair_count, S) ||
-         dev_alloc(b, &pair_pressure, S) || dev_alloc(b, reinterpret_cast<unsigned **>(&b->staging), 9 * S) ||
+         dev_alloc(b, &pair_rec, sphk::kFluidPlanes * S) || dev_alloc(b, reinterpret_cast<unsigned **>(&b->staging), 9 * S) ||
          dev_alloc(b, &b->gridcount, C) || dev_alloc(b, &b->celloffset, C) ||
          dev_alloc(b, &b->bins.tile_sums, tiles) || dev_alloc(b, &b->d_scenes, scenes) ||
          dev_alloc(b, &b->d_steps_left, scenes) || dev_alloc(b, &b->body_dynamic, sphk::kBodiesPerScene * scenes) ||
@@ -764,7 +764,7 @@ extern "C" int sph_batch_create(unsigned slots, unsigned cells, unsigned scenes,
         release(b);
         return -1;
     }
-    b->pairs = sphk::PairCache{pair_list, pair_count, pair_pressure, slots, 1};
+    b->pairs = sphk::PairCache{pair_list, pair_count, pair_rec, slots, 1};
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->side[0], hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&b->side[1], hipStreamNonBlocking) != hipSuccess ||

@@ -152,10 +152,16 @@ constexpr int kSolidList = 2;
 constexpr unsigned kListCurrent = 1u << 31;
 constexpr unsigned kListReference = 1u << 30;
 constexpr unsigned kListSlot = kListReference - 1u;   // solid lists need slots < 2^30
+// Fluid neighbour records, written by the density pass for the listed force
+// pass: per slot, two float4 planes, (position, density) and (velocity,
+// equation of state of the density), plane k of slot j at float4 index
+// k * stride + j, so that each gather reads one 16-byte-strided array.
+constexpr unsigned kFluidPlanes = 2;
+
 struct PairCache {
     unsigned *list;        // [kListCap * stride]
     unsigned *count;       // [stride]
-    float *pressure;       // [stride] equation of state of each slot's density (fluid lists)
+    float4 *rec;           // [kFluidPlanes * stride] fluid neighbour records
     unsigned stride;       // slots per list column (pnum, or the batch capacity)
     int valid;             // kFluidList / kSolidList: built from the current state; 0: stale
 };

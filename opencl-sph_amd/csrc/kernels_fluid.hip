@@ -165,6 +165,35 @@ struct RowBases {
     }
 };
 
+// One round of the fluid scan: kScanUnroll candidates from slot j0, loads
+// first (through a 64-bit address, so that the round's offsets fold into the
+// loads), then the range tests in slot order, appending ent + u to the
+// lane's LDS column at w for each hit.  CHECK_LEFT: the round reaches past
+// the row's end, and only its first `left` candidates are used.  CLAMP: the
+// column may fill up during the round; past its last entry, that entry is
+// overwritten (the list is then discarded, its count being past kListCap).
+template <bool CHECK_LEFT, bool CLAMP>
+__device__ __forceinline__ void scan_round(const float *__restrict__ pos, unsigned j0, unsigned left, V3 xi,
+                                           float r2_out, unsigned ent, unsigned short *&w,
+                                           unsigned short *wlast) {
+    const Packed3 *q = reinterpret_cast<const Packed3 *>(pos) + j0;
+    V3 xs[kScanUnroll];
+#pragma unroll
+    for (unsigned u = 0; u < kScanUnroll; ++u) {
+        const Packed3 t = q[u];
+        xs[u] = V3{t.x, t.y, t.z};
+    }
+#pragma unroll
+    for (unsigned u = 0; u < kScanUnroll; ++u) {
+        const float s = sqlength(xs[u] - xi);
+        const bool hit = CHECK_LEFT ? ((u < left) & !(s >= r2_out)) : !(s >= r2_out);
+        if (hit) {
+            *(CLAMP ? min(w, wlast) : w) = static_cast<unsigned short>(ent + u);
+            w += kListBlock;
+        }
+    }
+}
+
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs f, PairCache c) {
     __shared__ unsigned short buf[kListCap * kListBlock];
@@ -189,6 +218,11 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
         const unsigned rem = local - g.layer * zo;
         const unsigned yo = rem / g.rx;
         const unsigned xo = rem - g.rx * yo;
+        // next free entry of the lane's column; the column's last entry; the
+        // last position with room for a whole round
+        unsigned short *w = col;
+        unsigned short *const wlast = col + (kListCap - 1u) * kListBlock;
+        unsigned short *const wroom = col + (kListCap - kScanUnroll) * kListBlock;
 #pragma unroll 1
         for (unsigned r = 0; r < 9u; ++r) {
             unsigned b = 0, e = 0;
@@ -201,25 +235,21 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
             // not used), tested in slot order.  j0 - b is a multiple of
             // kScanUnroll, so entry (r << kRowBits) | (j0 + u - b) is ent + u.
             static_assert(kScanUnroll <= kScanPad, "scan rounds read past the row end");
-            for (unsigned j0 = b; j0 < e; j0 += kScanUnroll) {
-                V3 xs[kScanUnroll];
-                const unsigned off = j0 * 12u;
-#pragma unroll
-                for (unsigned u = 0; u < kScanUnroll; ++u) xs[u] = load3_at(pos, off + 12u * u);
-                const unsigned left = e - j0;   // candidates of this round inside the row
+            // Whole rounds need no end-of-row test; a round needs no cap test
+            // when every lane of the wave has room for all its candidates.
+            unsigned j0 = b;
+            for (; j0 + kScanUnroll <= e; j0 += kScanUnroll) {
                 const unsigned ent = (r << kRowBits) | (j0 - b);
-#pragma unroll
-                for (unsigned u = 0; u < kScanUnroll; ++u) {
-                    const float s = sqlength(xs[u] - xi);
-                    if ((u < left) & !(s >= r2_out)) {
-                        // past the cap the last entry is overwritten: the
-                        // list is then discarded (cnt > kListCap) anyway
-                        col[min(cnt, kListCap - 1u) * kListBlock] = static_cast<unsigned short>(ent + u);
-                        ++cnt;
-                    }
-                }
+                if (__all(w <= wroom)) scan_round<false, false>(pos, j0, 0u, xi, r2_out, ent, w, wlast);
+                else scan_round<false, true>(pos, j0, 0u, xi, r2_out, ent, w, wlast);
+            }
+            if (j0 < e) {
+                const unsigned ent = (r << kRowBits) | (j0 - b);
+                if (__all(w <= wroom)) scan_round<true, false>(pos, j0, e - j0, xi, r2_out, ent, w, wlast);
+                else scan_round<true, true>(pos, j0, e - j0, xi, r2_out, ent, w, wlast);
             }
         }
+        cnt = static_cast<unsigned>(w - col) / kListBlock;
         if (cnt > kListCap) fast = false;
     }
     float rho;
@@ -256,7 +286,9 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     f.density[i] = out;
     if (f.num_neighbours) f.num_neighbours[i] = nn;
     c.count[i] = cnt;
-    c.pressure[i] = pressure(p, out);
+    const V3 vi = load3(f.velocity, i);
+    c.rec[i] = make_float4(xi.x, xi.y, xi.z, out);
+    c.rec[static_cast<size_t>(c.stride) + i] = make_float4(vi.x, vi.y, vi.z, pressure(p, out));
 }
 
 // compute_density for solid programs, which also records the kSolidList
@@ -269,6 +301,36 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
 // the reference loop and mark their list as overflowed; lanes with more than
 // kListCap entries or a row longer than 1024 slots scan again without LDS.
 constexpr unsigned kSolidOffsetBits = 10;
+
+// One round of the solid scan: kSolidScanUnroll candidates from slot j0,
+// tested against the current and the reference position, appending
+// (ent + u) | current << 11 | reference << 10 for each candidate in either
+// range.  CHECK_LEFT and CLAMP as in scan_round.
+template <bool CHECK_LEFT, bool CLAMP>
+__device__ __forceinline__ void solid_scan_round(const float *__restrict__ pos, const float *__restrict__ opos,
+                                                 unsigned j0, unsigned left, V3 xi, V3 x0i, float r2_out,
+                                                 unsigned ent, unsigned short *&w, unsigned short *wlast) {
+    const Packed3 *q = reinterpret_cast<const Packed3 *>(pos) + j0;
+    const Packed3 *q0 = reinterpret_cast<const Packed3 *>(opos) + j0;
+    V3 xs[kSolidScanUnroll], x0s[kSolidScanUnroll];
+#pragma unroll
+    for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
+        const Packed3 t = q[u], t0 = q0[u];
+        xs[u] = V3{t.x, t.y, t.z};
+        x0s[u] = V3{t0.x, t0.y, t0.z};
+    }
+#pragma unroll
+    for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
+        const bool cur = !(sqlength(xs[u] - xi) >= r2_out);
+        const bool ref = !(sqlength(x0s[u] - x0i) >= r2_out);
+        const bool hit = CHECK_LEFT ? ((u < left) & (cur | ref)) : (cur | ref);
+        if (hit) {
+            *(CLAMP ? min(w, wlast) : w) =
+                static_cast<unsigned short>((ent + u) | (cur ? 1u << 11 : 0u) | (ref ? 1u << 10 : 0u));
+            w += kListBlock;
+        }
+    }
+}
 
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, FieldPtrs f, PairCache c) {
@@ -301,6 +363,9 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
             const unsigned rem = local - g.layer * zo;
             const unsigned yo = rem / g.rx;
             const unsigned xo = rem - g.rx * yo;
+            unsigned short *w = col;   // as in k_density_listed
+            unsigned short *const wlast = col + (kListCap - 1u) * kListBlock;
+            unsigned short *const wroom = col + (kListCap - kSolidScanUnroll) * kListBlock;
 #pragma unroll 1
             for (unsigned r = 0; r < 9u; ++r) {
                 unsigned b = 0, e = 0;
@@ -309,29 +374,22 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                 if (!inside) continue;
                 if (e - b > (1u << kSolidOffsetBits)) fast = false;
                 static_assert(kSolidScanUnroll <= kScanPad, "scan rounds read past the row end");
-                for (unsigned j0 = b; j0 < e; j0 += kSolidScanUnroll) {
-                    V3 xs[kSolidScanUnroll], x0s[kSolidScanUnroll];
-                    const unsigned off = j0 * 12u;
-#pragma unroll
-                    for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
-                        xs[u] = load3_at(pos, off + 12u * u);
-                        x0s[u] = load3_at(opos, off + 12u * u);
-                    }
-                    const unsigned left = e - j0;   // candidates of this round inside the row
-                    const unsigned ent = (r << kRowBits) | (j0 - b);   // j0 - b: a multiple of the unroll
-#pragma unroll
-                    for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
-                        const bool cur = !(sqlength(xs[u] - xi) >= r2_out);
-                        const bool ref = !(sqlength(x0s[u] - x0i) >= r2_out);
-                        if ((u < left) & (cur | ref)) {
-                            // past the cap the last entry is overwritten (list discarded)
-                            col[min(cnt, kListCap - 1u) * kListBlock] = static_cast<unsigned short>(
-                                (ent + u) | (cur ? 1u << 11 : 0u) | (ref ? 1u << 10 : 0u));
-                            ++cnt;
-                        }
-                    }
+                // j0 - b is a multiple of the unroll: entry (r << kRowBits) | (j0 + u - b) is ent + u
+                unsigned j0 = b;
+                for (; j0 + kSolidScanUnroll <= e; j0 += kSolidScanUnroll) {
+                    const unsigned ent = (r << kRowBits) | (j0 - b);
+                    if (__all(w <= wroom))
+                        solid_scan_round<false, false>(pos, opos, j0, 0u, xi, x0i, r2_out, ent, w, wlast);
+                    else solid_scan_round<false, true>(pos, opos, j0, 0u, xi, x0i, r2_out, ent, w, wlast);
+                }
+                if (j0 < e) {
+                    const unsigned ent = (r << kRowBits) | (j0 - b);
+                    if (__all(w <= wroom))
+                        solid_scan_round<true, false>(pos, opos, j0, e - j0, xi, x0i, r2_out, ent, w, wlast);
+                    else solid_scan_round<true, true>(pos, opos, j0, e - j0, xi, x0i, r2_out, ent, w, wlast);
                 }
             }
+            cnt = static_cast<unsigned>(w - col) / kListBlock;
             if (cnt > kListCap) fast = false;
         }
         Acc<KAHAN> acc;
@@ -461,8 +519,10 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
 }
 
 // compute_forces_fluids over the pair cache: the in-range list of each slot
-// and the pressures of the density pass (identical values to evaluating the
-// equation of state per pair).  Overflowed lists fall back to the scan.
+// and the (position, density) and (velocity, pressure) planes of the density
+// pass, two aligned 16-byte gathers per neighbour (the pressures are
+// identical values to evaluating the equation of state per pair).
+// Overflowed lists fall back to the scan.
 template <class PS, bool KAHAN, unsigned G>
 __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtrs f, PairCache c) {
     const unsigned i = ps.slot_xcd(blockDim.x);
@@ -470,13 +530,12 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
     const StepParams &p = ps.at(i);
     const KernelConsts kc = kconsts(p);
     const float r2o = p.r2_out;   // read once: with a scene table p is in global memory
-    const float *__restrict__ pos = f.position;
-    const float *__restrict__ vel = f.velocity;
-    const float *__restrict__ rho = f.density;
-    const float *__restrict__ prs = c.pressure;
-    const V3 xi = load3(pos, i), vi = load3(vel, i);
-    const float rhoi = rho[i];
-    const float pi = prs[i];
+    const float4 *__restrict__ xr = c.rec;
+    const float4 *__restrict__ vp = c.rec + c.stride;
+    const float4 ai = xr[i], bi = vp[i];
+    const V3 xi{ai.x, ai.y, ai.z}, vi{bi.x, bi.y, bi.z};
+    const float rhoi = ai.w;
+    const float pi = bi.w;
     const float visc = p.viscosity;   // read once: with a scene table p is in global memory
     FluidForce<KAHAN> acc;
     const unsigned cnt = c.count[i];
@@ -484,20 +543,23 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
         // List entries are in range by construction (same positions, same
         // distance expression), so the pair terms need no range test here.
         struct Nbr {
-            V3 x, v;
-            float rho, p;
+            float4 a, b;
         };
         for_each_listed<G>(
-            c, i, cnt, [&](unsigned j) { return Nbr{load3(pos, j), load3(vel, j), load1(rho, j), load1(prs, j)}; },
+            c, i, cnt, [&](unsigned j) { return Nbr{xr[j], vp[j]}; },
             [&](unsigned j, const Nbr &n) {
-                if (j != i) acc.pair(visc, kc, xi, vi, rhoi, pi, n.x, n.v, n.rho, n.p);
+                if (j != i)
+                    acc.pair(visc, kc, xi, vi, rhoi, pi, V3{n.a.x, n.a.y, n.a.z}, V3{n.b.x, n.b.y, n.b.z}, n.a.w,
+                             n.b.w);
             });
     } else {
         for_each_candidate(p.grid, p.p0, p.p0 + p.n, f.gridcell[i], f.celloffset, f.gridcount, [&](unsigned j) {
             if (j == i) return;
-            const V3 xj = load3(pos, j);
+            const float4 a = xr[j];
+            const V3 xj{a.x, a.y, a.z};
             if (sqlength(xj - xi) >= r2o) return;   // length > h, without the square root
-            acc.pair(visc, kc, xi, vi, rhoi, pi, xj, load3(vel, j), rho[j], prs[j]);
+            const float4 b = vp[j];
+            acc.pair(visc, kc, xi, vi, rhoi, pi, xj, V3{b.x, b.y, b.z}, a.w, b.w);
         });
     }
     store3(f.force, i, acc.finish(p, xi));

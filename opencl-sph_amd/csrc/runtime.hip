@@ -526,7 +526,7 @@ void release(psdata_hip *h) {
     }
     unsigned *scratch[] = {h->slot_id[0], h->slot_id[1], h->perm, h->bins.key, h->bins.local, h->bins.tmp_id,
                            h->bins.tmp_src, h->bins.tmp_key, h->bins.tile_sums, h->pairs.list, h->pairs.count,
-                           reinterpret_cast<unsigned *>(h->pairs.pressure)};
+                           reinterpret_cast<unsigned *>(h->pairs.rec)};
     for (unsigned *p : scratch)
         if (p) hipFree(p);
     if (h->staging) hipFree(h->staging);
@@ -982,7 +982,8 @@ extern "C" int create_psdata_hip(psdata *data, const char *file_list, psdata_hip
          h->has_grid &&
          (hipMalloc(reinterpret_cast<void **>(&h->pairs.list), pn4 * sphk::kListCap) != hipSuccess ||
           hipMalloc(reinterpret_cast<void **>(&h->pairs.count), pn4) != hipSuccess ||
-          hipMalloc(reinterpret_cast<void **>(&h->pairs.pressure), pn4) != hipSuccess))) {
+          (h->kset.fluids &&
+           hipMalloc(reinterpret_cast<void **>(&h->pairs.rec), sphk::kFluidPlanes * 4 * pn4) != hipSuccess)))) {
         release(h);
         return fail("out of device memory (scratch)");
     }
