@@ -66,6 +66,7 @@ def grid_pair_counts(system, scene):
 # (a pass may launch several kernels with that prefix: their traffic adds up).
 KERNEL_SYMBOLS = {"compute_density": "k_density_listed<sphk::SingleScene",
                   "compute_forces_fluids": "k_forces_fluids_listed<sphk::SingleScene",
+                  "compute_forces_fluids+step_forward": "k_forces_fluids_listed<sphk::SingleScene",
                   "bin_rank_place": "k_rank_place", "step_forward": "k_step_forward<sphk::SingleScene",
                   "bin_scatter": "k_scatter_unsorted", "bin_key_count": "k_key_count"}
 BATCH_KERNEL_SYMBOLS = {
@@ -353,7 +354,8 @@ def dam_break_4m_block(args):
 def _brief(rep):
     keep = ("requests", "particle_steps", "wall_s", "particle_steps_per_s", "latency_p50_s", "latency_p99_s",
             "latency_mean_s", "latency_max_s", "peak_resident_slots", "peak_resident_scenes", "batch_steps",
-            "graph_captures", "graph_launches", "pool_bytes", "host_phases_s")
+            "graph_captures", "graph_launches", "pool_bytes", "host_phases_s", "gpu_busy_lower_bound",
+            "prepare_wait_share")
     return {k: (round(v, 5) if isinstance(v, float) else v) for k, v in rep.items() if k in keep}
 
 

@@ -203,7 +203,9 @@ hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipSt
 // Density that also fills the pair cache (fluid programs).
 hipError_t launch_compute_density_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
                                          hipStream_t s);
-hipError_t launch_forces_fluids_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
+// integrate: also step_forward, in the same kernel (the pass reads every
+// neighbour from the density pass's planes).
+hipError_t launch_forces_fluids_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c, bool integrate,
                                        hipStream_t s);
 // Density that also fills a kSolidList pair cache (solid programs).
 hipError_t launch_compute_density_solid_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,

@@ -518,12 +518,34 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids(PS ps, FieldPtrs f) {
     store3(f.force, i, acc.finish(p, xi));
 }
 
+__device__ __forceinline__ bool is_static_body(const StepParams &p, const FieldPtrs &f, unsigned i) {
+    if (!f.body_number) return false;
+    const unsigned b = f.body_number[i];
+    if (b == 0u || !f.body_dynamic || b > p.body_count) return false;
+    return f.body_dynamic[p.body_base + b - 1u] == 0u;
+}
+
+// step_forward of slot i (kernels/common.cl:691-722) from its force and its
+// current velocity and position.
+__device__ __forceinline__ void integrate(const StepParams &p, const FieldPtrs &f, unsigned i, V3 force, V3 v,
+                                          V3 x) {
+    if (is_static_body(p, f, i)) return;
+    const V3 g = V3{p.gravity[0], p.gravity[1], p.gravity[2]};
+    const V3 a = force / p.mass + g;
+    if (f.acceleration) store3(f.acceleration, i, a);
+    const V3 vn = v + p.dt * a;
+    store3(f.position, i, x + p.dt * vn);
+    store3(f.velocity, i, vn);
+}
+
 // compute_forces_fluids over the pair cache: the in-range list of each slot
 // and the (position, density) and (velocity, pressure) planes of the density
 // pass, two aligned 16-byte gathers per neighbour (the pressures are
 // identical values to evaluating the equation of state per pair).
-// Overflowed lists fall back to the scan.
-template <class PS, bool KAHAN, unsigned G>
+// Overflowed lists fall back to the scan.  INTEGRATE: also step_forward
+// for the slot, in place: every neighbour's position and velocity come from
+// the density pass's planes, so no lane reads what another lane updates.
+template <class PS, bool KAHAN, unsigned G, bool INTEGRATE>
 __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtrs f, PairCache c) {
     const unsigned i = ps.slot_xcd(blockDim.x);
     if (!ps.active(i)) return;
@@ -562,14 +584,9 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
             acc.pair(visc, kc, xi, vi, rhoi, pi, xj, V3{b.x, b.y, b.z}, a.w, b.w);
         });
     }
-    store3(f.force, i, acc.finish(p, xi));
-}
-
-__device__ __forceinline__ bool is_static_body(const StepParams &p, const FieldPtrs &f, unsigned i) {
-    if (!f.body_number) return false;
-    const unsigned b = f.body_number[i];
-    if (b == 0u || !f.body_dynamic || b > p.body_count) return false;
-    return f.body_dynamic[p.body_base + b - 1u] == 0u;
+    const V3 fi = acc.finish(p, xi);
+    store3(f.force, i, fi);
+    if (INTEGRATE) integrate(p, f, i, fi, vi, xi);
 }
 
 template <class PS>
@@ -578,14 +595,7 @@ __global__ void __launch_bounds__(kBlock) k_step_forward(PS ps, FieldPtrs f) {
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
     if (is_static_body(p, f, i)) return;
-    const V3 g = V3{p.gravity[0], p.gravity[1], p.gravity[2]};
-    const V3 a = load3(f.force, i) / p.mass + g;
-    if (f.acceleration) store3(f.acceleration, i, a);
-    const V3 v = load3(f.velocity, i);
-    const V3 vn = v + p.dt * a;
-    const V3 x = load3(f.position, i);
-    store3(f.position, i, x + p.dt * vn);
-    store3(f.velocity, i, vn);
+    integrate(p, f, i, load3(f.force, i), load3(f.velocity, i), load3(f.position, i));
 }
 
 template <class PS>
@@ -694,26 +704,32 @@ hipError_t density_solid_listed(const PS &ps, const FieldPtrs &f, const PairCach
     return hipGetLastError();
 }
 
-template <class PS>
-hipError_t forces_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated, hipStream_t s) {
-    if (!ps.has_work()) return hipSuccess;
+template <class PS, bool INTEGRATE>
+hipError_t forces_listed_g(const PS &ps, const FieldPtrs &f, const PairCache &c, bool kahan, hipStream_t s) {
     const unsigned nb = neighbour_block();
     const dim3 g = ps.launch_blocks(nb, true), b(nb);
-    const bool kahan = (compensated & kKahanForces) != 0;
     switch (fluid_pair_group()) {
     case 12:
-        if (kahan) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true, 12>), g, b, 0, s, ps, f, c);
-        else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false, 12>), g, b, 0, s, ps, f, c);
+        if (kahan) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true, 12, INTEGRATE>), g, b, 0, s, ps, f, c);
+        else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false, 12, INTEGRATE>), g, b, 0, s, ps, f, c);
         break;
     case 16:
-        if (kahan) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true, 16>), g, b, 0, s, ps, f, c);
-        else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false, 16>), g, b, 0, s, ps, f, c);
+        if (kahan) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true, 16, INTEGRATE>), g, b, 0, s, ps, f, c);
+        else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false, 16, INTEGRATE>), g, b, 0, s, ps, f, c);
         break;
     default:
-        if (kahan) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true, kPairGroup>), g, b, 0, s, ps, f, c);
-        else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false, kPairGroup>), g, b, 0, s, ps, f, c);
+        if (kahan) hipLaunchKernelGGL((k_forces_fluids_listed<PS, true, kPairGroup, INTEGRATE>), g, b, 0, s, ps, f, c);
+        else hipLaunchKernelGGL((k_forces_fluids_listed<PS, false, kPairGroup, INTEGRATE>), g, b, 0, s, ps, f, c);
     }
     return hipGetLastError();
+}
+
+template <class PS>
+hipError_t forces_listed(const PS &ps, const FieldPtrs &f, const PairCache &c, int compensated, bool integrate,
+                         hipStream_t s) {
+    if (!ps.has_work()) return hipSuccess;
+    const bool kahan = (compensated & kKahanForces) != 0;
+    return integrate ? forces_listed_g<PS, true>(ps, f, c, kahan, s) : forces_listed_g<PS, false>(ps, f, c, kahan, s);
 }
 
 template <class PS>
@@ -741,9 +757,9 @@ hipError_t launch_compute_density_listed(const StepParams &p, const FieldPtrs &f
     return density_listed(SingleScene{p}, f, c, p.compensated, s);
 }
 
-hipError_t launch_forces_fluids_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
+hipError_t launch_forces_fluids_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c, bool integrate,
                                        hipStream_t s) {
-    return forces_listed(SingleScene{p}, f, c, p.compensated, s);
+    return forces_listed(SingleScene{p}, f, c, p.compensated, integrate, s);
 }
 
 hipError_t launch_compute_density_solid_listed(const StepParams &p, const FieldPtrs &f, const PairCache &c,
@@ -814,7 +830,7 @@ hipError_t launch_batch_init_original_position(const BatchTable &b, SceneFilter 
 
 hipError_t launch_batch_forces_fluids_listed(const BatchTable &b, SceneFilter sel, const FieldPtrs &f,
                                              const PairCache &c, int compensated, hipStream_t s) {
-    return forces_listed(SceneTable{b, sel}, f, c, compensated, s);
+    return forces_listed(SceneTable{b, sel}, f, c, compensated, false, s);
 }
 
 hipError_t launch_batch_step_forward(const BatchTable &b, SceneFilter sel, const FieldPtrs &f, hipStream_t s) {

@@ -460,7 +460,7 @@ int run_kernel(psdata_hip *h, const std::string &internal) {
         rc = timed(h, "compute_density", [&] { return sphk::launch_compute_density(p, f, s); });
     else if (internal == "compute_forces_fluids" && cache.valid == sphk::kFluidList)
         rc = timed(h, "compute_forces_fluids",
-                   [&] { return sphk::launch_forces_fluids_listed(p, f, cache, s); });
+                   [&] { return sphk::launch_forces_fluids_listed(p, f, cache, false, s); });
     else if (internal == "compute_forces_fluids")
         rc = timed(h, "compute_forces_fluids", [&] { return sphk::launch_forces_fluids(p, f, s); });
     else if (internal == "step_forward")
@@ -495,6 +495,27 @@ int run_kernel(psdata_hip *h, const std::string &internal) {
     if (rc) return -1;
     mark_written(h, k);
     if (internal == "compute_density" && listed) h->pairs.valid = h->kset.fluids ? sphk::kFluidList : sphk::kSolidList;
+    return 0;
+}
+
+// compute_forces_fluids then step_forward as one kernel, over a valid fluid
+// pair cache: the force pass reads every neighbour from the density pass's
+// planes, so it can move each particle in place once its force is known.
+// Field bookkeeping as for the two kernels run one after the other.
+int run_forces_and_step(psdata_hip *h) {
+    const KernelSpec kf = spec_for(h, "compute_forces_fluids"), ks = spec_for(h, "step_forward");
+    if (check_and_prepare(h, kf)) return -1;
+    mark_written(h, kf);   // force: written in full by the launch below
+    if (check_and_prepare(h, ks)) return -1;
+    sphk::StepParams p;
+    if (build_params(h, &p)) return -1;
+    const sphk::FieldPtrs f = field_ptrs(h);
+    const sphk::PairCache cache = h->pairs;
+    h->pairs.valid = 0;   // the particles move
+    if (timed(h, "compute_forces_fluids+step_forward",
+              [&] { return sphk::launch_forces_fluids_listed(p, f, cache, true, h->stream); }))
+        return -1;
+    mark_written(h, ks);
     return 0;
 }
 
@@ -1119,8 +1140,15 @@ extern "C" int step_device_hip(psdata_hip *h, const char *system, unsigned int n
         if (resolve_kernel(h, seq[k], &internal[k])) return -1;
     for (unsigned s = 0; s < num_steps; ++s) {
         if (compute_bins(h)) return -1;
-        for (const std::string &k : internal)
-            if (run_kernel(h, k)) return -1;
+        for (size_t k = 0; k < internal.size(); ++k) {
+            if (internal[k] == "compute_forces_fluids" && k + 1 < internal.size() &&
+                internal[k + 1] == "step_forward" && h->pairs.valid == sphk::kFluidList) {
+                if (run_forces_and_step(h)) return -1;
+                ++k;
+                continue;
+            }
+            if (run_kernel(h, internal[k])) return -1;
+        }
     }
     return 0;
 }

@@ -22,6 +22,12 @@ _FLUID_KERNELS = {
     "compute_forces_fluids": (44, 8, 8, 31),
     "step_forward": (72, 0, 0, 0),
 }
+# Passes that run two of the above as one kernel (not counted again in the
+# step totals): the force pass with the integrator fused in reads x, v and
+# density and writes force, acceleration, x and v.
+_FUSED_KERNELS = {
+    "compute_forces_fluids+step_forward": (76, 8, 8, 31),
+}
 
 
 # Batch passes: (systems the pass runs for, bytes per active particle, bytes
@@ -99,7 +105,7 @@ def batch_bytes_per_launch(name, particle_steps_by_system, launches, cell_steps=
 
 
 def kernel_model(name):
-    return _FLUID_KERNELS.get(name)
+    return _FLUID_KERNELS.get(name) or _FUSED_KERNELS.get(name)
 
 
 def bytes_per_launch(name, n, cells):

@@ -263,8 +263,14 @@ def serve_batched(requests, hbm_budget=200e9, slots=None, cells=None, scene_entr
         for b in batches:
             b.free()
     t_first = requests[0].arrival if realtime else 0.0
+    wall = max(t_end - min(t_first, t_end), 1e-9)
     return _summary("batched", requests, done, min(t_first, t_end), t_end,
-                    {"pool_bytes": pool_bytes, "pool_slots": slots, "pool_cells": cells, "pools": npool,
+                    {# the host blocked on the device for `wait`, so the device was executing
+                     # at least that share of the wall time; prepare_wait: the host waited for
+                     # scene preparation (psdata builds) with the device possibly idle
+                     "gpu_busy_lower_bound": round(phase["wait"] / wall, 4),
+                     "prepare_wait_share": round(phase["prepare_wait"] / wall, 4),
+                     "pool_bytes": pool_bytes, "pool_slots": slots, "pool_cells": cells, "pools": npool,
                      "peak_resident_slots": peak_slots, "peak_resident_scenes": peak_scenes,
                      "batch_steps": busy_steps, "graph_captures": sum(i["graph_captures"] for i in infos),
                      "graph_launches": sum(i["graph_launches"] for i in infos), "graph_steps": graph_steps,
