@@ -141,7 +141,8 @@ hipError_t launch_scatter_words(const void *src, void *dst, const unsigned *map,
 // list[k * stride + i] in the reference's visiting order (self included);
 // count[i] > kListCap means the list overflowed (or was not built for slot
 // i) and readers scan the candidates instead.
-//   kFluidList: the slots j with length(x_j - x_i) <= h.
+//   kFluidList: the slots j with length(x_j - x_i) <= h, as 16-bit entries
+//     (see kFluidPlanes).
 //   kSolidList: the slots j with length(x_j - x_i) <= h (flag
 //     kListCurrent) or length(x0_j - x0_i) <= h (flag kListReference), the
 //     union of the current and reference neighbourhoods that the solid passes
@@ -153,10 +154,15 @@ constexpr unsigned kListCurrent = 1u << 31;
 constexpr unsigned kListReference = 1u << 30;
 constexpr unsigned kListSlot = kListReference - 1u;   // solid lists need slots < 2^30
 // Fluid neighbour records, written by the density pass for the listed force
-// pass: per slot, two float4 planes, (position, density) and (velocity,
-// equation of state of the density), plane k of slot j at float4 index
-// k * stride + j, so that each gather reads one 16-byte-strided array.
-constexpr unsigned kFluidPlanes = 2;
+// pass: per slot, float4 planes (position, density), (velocity, equation of
+// state of the density) and the slot's list bases (three slots as bits),
+// plane k of slot j at float4 index k * stride + j, so that each gather
+// reads one 16-byte-strided array.  A fluid list holds 16-bit entries
+// (dz group << 14 | offset from that group's base), entries 2m and 2m + 1
+// of slot i in the low and high half of list[m * stride + i]: within the
+// words a 32-bit list of slot i would use, so that in a batch the fluid and
+// solid lists of different slots never share a word.
+constexpr unsigned kFluidPlanes = 3;
 
 struct PairCache {
     unsigned *list;        // [kListCap * stride]

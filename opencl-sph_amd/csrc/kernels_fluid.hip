@@ -194,6 +194,20 @@ __device__ __forceinline__ void scan_round(const float *__restrict__ pos, unsign
     }
 }
 
+// First slots of the three dz groups of neighbour rows of a lane (rows 3g
+// to 3g + 2: the rows of one z layer of cells, in slot order), kept in
+// registers between the scan and the kernel sum and stored with the list.
+// A fluid list entry is (g << kGroupBits) | (j - base of group g), 16 bits.
+constexpr unsigned kGroupBits = 14;
+__device__ __forceinline__ unsigned group_slot(unsigned ent, unsigned b0, unsigned b1, unsigned b2) {
+    const unsigned g = ent >> kGroupBits;
+    return (g == 0u ? b0 : (g == 1u ? b1 : b2)) + (ent & ((1u << kGroupBits) - 1u));
+}
+struct GroupBases {
+    unsigned b0 = 0, b1 = 0, b2 = 0;
+    __device__ __forceinline__ unsigned slot(unsigned ent) const { return group_slot(ent, b0, b1, b2); }
+};
+
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs f, PairCache c) {
     __shared__ unsigned short buf[kListCap * kListBlock];
@@ -211,7 +225,7 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     unsigned short *col = buf + threadIdx.x;
     unsigned cnt = 0;
     bool fast = !inear && key >= g.c0 && key - g.c0 < g.ncells;
-    RowBases rows;
+    GroupBases gb;
     if (fast) {
         const unsigned local = key - g.c0;
         const unsigned zo = local / g.layer;
@@ -223,28 +237,38 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
         unsigned short *w = col;
         unsigned short *const wlast = col + (kListCap - 1u) * kListBlock;
         unsigned short *const wroom = col + (kListCap - kScanUnroll) * kListBlock;
+        unsigned gbase = 0;   // first slot of the current group's first row inside the grid
+        bool gset = false;
 #pragma unroll 1
         for (unsigned r = 0; r < 9u; ++r) {
+            const unsigned gi = r / 3u;
+            if (r == 3u * gi) gset = false;
             unsigned b = 0, e = 0;
             const bool inside = row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e);
-            rows.push(b);
             if (!inside) continue;
-            if (e - b > (1u << kRowBits)) fast = false;
+            if (!gset) {
+                gbase = b;
+                gset = true;
+                if (gi == 0u) gb.b0 = b;
+                else if (gi == 1u) gb.b1 = b;
+                else gb.b2 = b;
+            }
+            if (e - gbase > (1u << kGroupBits)) fast = false;
             // kScanUnroll candidates per round, loads first (slots past the
             // row's end are read, within the buffer's kScanPad padding, and
-            // not used), tested in slot order.  j0 - b is a multiple of
-            // kScanUnroll, so entry (r << kRowBits) | (j0 + u - b) is ent + u.
+            // not used), tested in slot order: entry ebase + (j0 - b) + u.
             static_assert(kScanUnroll <= kScanPad, "scan rounds read past the row end");
+            const unsigned ebase = (gi << kGroupBits) + (b - gbase);
             // Whole rounds need no end-of-row test; a round needs no cap test
             // when every lane of the wave has room for all its candidates.
             unsigned j0 = b;
             for (; j0 + kScanUnroll <= e; j0 += kScanUnroll) {
-                const unsigned ent = (r << kRowBits) | (j0 - b);
+                const unsigned ent = ebase + (j0 - b);
                 if (__all(w <= wroom)) scan_round<false, false>(pos, j0, 0u, xi, r2_out, ent, w, wlast);
                 else scan_round<false, true>(pos, j0, 0u, xi, r2_out, ent, w, wlast);
             }
             if (j0 < e) {
-                const unsigned ent = (r << kRowBits) | (j0 - b);
+                const unsigned ent = ebase + (j0 - b);
                 if (__all(w <= wroom)) scan_round<true, false>(pos, j0, e - j0, xi, r2_out, ent, w, wlast);
                 else scan_round<true, true>(pos, j0, e - j0, xi, r2_out, ent, w, wlast);
             }
@@ -255,15 +279,19 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     float rho;
     unsigned nn = 0;
     if (fast) {
+        // the list keeps the LDS entries as they are, two per word of the
+        // slot's own column (see kFluidPlanes), with the group bases
+        static_assert(kPairGroup % 2 == 0, "entries are stored in pairs");
         Acc<KAHAN> acc;
         for (unsigned k0 = 0; k0 < cnt; k0 += kPairGroup) {
-            unsigned js[kPairGroup];
+            unsigned es[kPairGroup], js[kPairGroup];
 #pragma unroll
             for (unsigned q = 0; q < kPairGroup; ++q) {
                 js[q] = i;
+                es[q] = 0;
                 if (k0 + q < cnt) {
-                    const unsigned ent = col[(k0 + q) * kListBlock];
-                    js[q] = rows.pick(ent >> kRowBits) + (ent & ((1u << kRowBits) - 1u));
+                    es[q] = col[(k0 + q) * kListBlock];
+                    js[q] = gb.slot(es[q]);
                 }
             }
             V3 xs[kPairGroup];
@@ -275,20 +303,27 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
                 const float rr = length(xs[q] - xi);
                 acc.add(w_poly6(rr, kc));
                 if (rr < kc.h) ++nn;
-                c.list[static_cast<size_t>(k0 + q) * c.stride + i] = js[q];
             }
+#pragma unroll
+            for (unsigned q = 0; q < kPairGroup; q += 2)
+                if (k0 + q < cnt)
+                    c.list[static_cast<size_t>((k0 + q) / 2u) * c.stride + i] = es[q] | (es[q + 1] << 16);
         }
         rho = acc.value();
     } else {
-        rho = density_scan<KAHAN, true>(p, f, i, xi, inear, &nn, c.list, c.stride, &cnt);
+        // the force pass scans for these lanes (no list)
+        rho = density_scan<KAHAN, true>(p, f, i, xi, inear, &nn, nullptr, 0, nullptr);
+        cnt = kListCap + 1u;
     }
     const float out = rho * p.mass;
     f.density[i] = out;
     if (f.num_neighbours) f.num_neighbours[i] = nn;
     c.count[i] = cnt;
     const V3 vi = load3(f.velocity, i);
+    const size_t st = c.stride;
     c.rec[i] = make_float4(xi.x, xi.y, xi.z, out);
-    c.rec[static_cast<size_t>(c.stride) + i] = make_float4(vi.x, vi.y, vi.z, pressure(p, out));
+    c.rec[st + i] = make_float4(vi.x, vi.y, vi.z, pressure(p, out));
+    c.rec[2 * st + i] = make_float4(__uint_as_float(gb.b0), __uint_as_float(gb.b1), __uint_as_float(gb.b2), 0.0f);
 }
 
 // compute_density for solid programs, which also records the kSolidList
@@ -567,8 +602,13 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
         struct Nbr {
             float4 a, b;
         };
-        for_each_listed<G>(
-            c, i, cnt, [&](unsigned j) { return Nbr{xr[j], vp[j]}; },
+        // the slot's list bases, as values (the decode is a select on them)
+        const float4 gbf = c.rec[2 * static_cast<size_t>(c.stride) + i];
+        const unsigned b0 = __float_as_uint(gbf.x), b1 = __float_as_uint(gbf.y), b2 = __float_as_uint(gbf.z);
+        for_each_listed16<G>(
+            c.list, c.stride, i, cnt,
+            [b0, b1, b2](unsigned ent) { return group_slot(ent, b0, b1, b2); },
+            [&](unsigned j) { return Nbr{xr[j], vp[j]}; },
             [&](unsigned j, const Nbr &n) {
                 if (j != i)
                     acc.pair(visc, kc, xi, vi, rhoi, pi, V3{n.a.x, n.a.y, n.a.z}, V3{n.b.x, n.b.y, n.b.z}, n.a.w,

@@ -78,6 +78,43 @@ __device__ __forceinline__ void for_each_listed(const PairCache &c, unsigned i, 
     }
 }
 
+// for_each_listed over a list of 16-bit entries (the fluid list: entries
+// 2m and 2m + 1 of slot i in the low and high half of word m * stride + i),
+// each turned into its slot by decode(entry).  G is even.  Words are read
+// without a test (the column holds kListCap words, and at most cnt + G
+// entries, cnt <= kListCap, are fetched); entries at or past cnt become i by
+// a select, so that no branch separates the loads.
+template <unsigned G, typename Decode, typename Load, typename Use>
+__device__ __forceinline__ void for_each_listed16(const unsigned *__restrict__ list, size_t stride, unsigned i,
+                                                  unsigned cnt, Decode &&decode, Load &&load, Use &&use) {
+    static_assert(G % 2 == 0, "entries are read in pairs");
+    static_assert((kListCap + G) / 2 <= kListCap, "fetched words stay in the slot's column");
+    if (cnt == 0) return;
+    const unsigned *__restrict__ col = list + i;
+    auto fetch = [&](unsigned k, unsigned *out) {   // entries k, k + 1 (k even)
+        const unsigned w = col[(k / 2u) * stride];
+        const unsigned a = decode(w & 0xffffu), b = decode(w >> 16);
+        out[0] = k < cnt ? a : i;
+        out[1] = k + 1u < cnt ? b : i;
+    };
+    unsigned e[G];
+#pragma unroll
+    for (unsigned g = 0; g < G; g += 2) fetch(g, e + g);
+    for (unsigned k0 = 0; k0 < cnt; k0 += G) {
+        decltype(load(0u)) d[G];
+#pragma unroll
+        for (unsigned g = 0; g < G; ++g) d[g] = load(e[g]);
+        unsigned nx[G];
+#pragma unroll
+        for (unsigned g = 0; g < G; g += 2) fetch(k0 + G + g, nx + g);
+#pragma unroll
+        for (unsigned g = 0; g < G; ++g)
+            if (k0 + g < cnt) use(e[g], d[g]);
+#pragma unroll
+        for (unsigned g = 0; g < G; ++g) e[g] = nx[g];
+    }
+}
+
 __device__ __forceinline__ V3 plane_point(const StepParams &p, unsigned s) {
     return V3{p.planes[s][0], p.planes[s][1], p.planes[s][2]};
 }
