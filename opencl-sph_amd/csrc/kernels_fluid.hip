@@ -118,14 +118,13 @@ __global__ void __launch_bounds__(kBlock) k_density(PS ps, FieldPtrs f) {
 // compute_density for fluid programs, in two phases per lane:
 //   1. scan the candidates with the square-root-free range test and compact
 //      the in-range slots into this lane's column of an LDS buffer, as 16-bit
-//      entries (row number << 12 | offset from the row's first slot);
+//      entries (dz group << 14 | offset from the group's first slot);
 //   2. evaluate the kernel sum over the compacted slots (every lane busy),
 //      writing them out as the pair cache's neighbour list.
 // The summation order is the candidate order, as in the reference.  Lanes
 // near a mirror plane, with more than kListCap in-range candidates or with a
-// row longer than 4096 slots take the reference loop instead.
+// dz group spanning more than 16384 slots take the reference loop instead.
 constexpr unsigned kListBlock = 128;
-constexpr unsigned kRowBits = 12;
 
 // First slot of neighbour row r (0..8, (dz, dy) = (r / 3 - 1, r % 3 - 1))
 // around the cell (xo, yo, zo) of the scene's grid, and one past its last
@@ -144,26 +143,6 @@ __device__ __forceinline__ bool row_range(const GridGeom &g, unsigned lo, unsign
     *e = min(max(celloffset[row + x1] + gridcount[row + x1], lo), hi);
     return true;
 }
-
-// First slots of the 9 neighbour rows of a lane, kept in registers between
-// the scan and the kernel sum: the scan pushes them in row order (every row,
-// inside the grid or not), the sum picks the row of each list entry with a
-// select tree instead of reloading celloffset.
-struct RowBases {
-    unsigned b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    __device__ __forceinline__ void push(unsigned v) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) b[k] = b[k + 1];
-        b[8] = v;
-    }
-    __device__ __forceinline__ unsigned pick(unsigned r) const {   // r in 0..8
-        const bool o1 = r & 1u, o2 = r & 2u, o4 = r & 4u;
-        const unsigned a01 = o1 ? b[1] : b[0], a23 = o1 ? b[3] : b[2];
-        const unsigned a45 = o1 ? b[5] : b[4], a67 = o1 ? b[7] : b[6];
-        const unsigned a03 = o2 ? a23 : a01, a47 = o2 ? a67 : a45;
-        return r >= 8u ? b[8] : (o4 ? a47 : a03);
-    }
-};
 
 // One round of the fluid scan: kScanUnroll candidates from slot j0, loads
 // first (through a 64-bit address, so that the round's offsets fold into the
@@ -329,17 +308,18 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
 // compute_density for solid programs, which also records the kSolidList
 // neighbour list (current or reference distance within h, with flags) in
 // candidate order.  Two phases per lane as in k_density_listed: the scan
-// compacts candidates in either range into LDS as 16-bit entries (row << 12
-// | current << 11 | reference << 10 | offset in row), then the kernel sum
-// runs over the compacted entries.  The range tests are the
-// square-root-free form of length(d) <= h.  Lanes near a mirror plane take
-// the reference loop and mark their list as overflowed; lanes with more than
-// kListCap entries or a row longer than 1024 slots scan again without LDS.
-constexpr unsigned kSolidOffsetBits = 10;
+// compacts candidates in either range into LDS as 16-bit entries (dz group
+// << 14 | current << 13 | reference << 12 | offset from the group's base),
+// then the kernel sum runs over the compacted entries.  The range tests are
+// the square-root-free form of length(d) <= h.  Lanes near a mirror plane
+// take the reference loop and mark their list as overflowed; lanes with more
+// than kListCap entries or a group spanning more than 4096 slots scan again
+// without LDS.
+constexpr unsigned kSolidOffsetBits = 12;
 
 // One round of the solid scan: kSolidScanUnroll candidates from slot j0,
 // tested against the current and the reference position, appending
-// (ent + u) | current << 11 | reference << 10 for each candidate in either
+// (ent + u) | current << 13 | reference << 12 for each candidate in either
 // range.  CHECK_LEFT and CLAMP as in scan_round.
 template <bool CHECK_LEFT, bool CLAMP>
 __device__ __forceinline__ void solid_scan_round(const float *__restrict__ pos, const float *__restrict__ opos,
@@ -361,7 +341,7 @@ __device__ __forceinline__ void solid_scan_round(const float *__restrict__ pos, 
         const bool hit = CHECK_LEFT ? ((u < left) & (cur | ref)) : (cur | ref);
         if (hit) {
             *(CLAMP ? min(w, wlast) : w) =
-                static_cast<unsigned short>((ent + u) | (cur ? 1u << 11 : 0u) | (ref ? 1u << 10 : 0u));
+                static_cast<unsigned short>((ent + u) | (cur ? 1u << 13 : 0u) | (ref ? 1u << 12 : 0u));
             w += kListBlock;
         }
     }
@@ -391,7 +371,7 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
     } else {
         const V3 x0i = load3(opos, i);
         bool fast = key >= g.c0 && key - g.c0 < g.ncells;
-        RowBases rows;
+        unsigned b0 = 0, b1 = 0, b2 = 0;   // dz group bases, as in k_density_listed
         if (fast) {
             const unsigned local = key - g.c0;
             const unsigned zo = local / g.layer;
@@ -401,24 +381,35 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
             unsigned short *w = col;   // as in k_density_listed
             unsigned short *const wlast = col + (kListCap - 1u) * kListBlock;
             unsigned short *const wroom = col + (kListCap - kSolidScanUnroll) * kListBlock;
+            unsigned gbase = 0;
+            bool gset = false;
 #pragma unroll 1
             for (unsigned r = 0; r < 9u; ++r) {
+                const unsigned gi = r / 3u;
+                if (r == 3u * gi) gset = false;
                 unsigned b = 0, e = 0;
                 const bool inside = row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e);
-                rows.push(b);
                 if (!inside) continue;
-                if (e - b > (1u << kSolidOffsetBits)) fast = false;
+                if (!gset) {
+                    gbase = b;
+                    gset = true;
+                    if (gi == 0u) b0 = b;
+                    else if (gi == 1u) b1 = b;
+                    else b2 = b;
+                }
+                if (e - gbase > (1u << kSolidOffsetBits)) fast = false;
                 static_assert(kSolidScanUnroll <= kScanPad, "scan rounds read past the row end");
-                // j0 - b is a multiple of the unroll: entry (r << kRowBits) | (j0 + u - b) is ent + u
+                // entry of slot j: (gi << kGroupBits) + (j - gbase), ent + u in a round from j0
+                const unsigned ebase = (gi << kGroupBits) + (b - gbase);
                 unsigned j0 = b;
                 for (; j0 + kSolidScanUnroll <= e; j0 += kSolidScanUnroll) {
-                    const unsigned ent = (r << kRowBits) | (j0 - b);
+                    const unsigned ent = ebase + (j0 - b);
                     if (__all(w <= wroom))
                         solid_scan_round<false, false>(pos, opos, j0, 0u, xi, x0i, r2_out, ent, w, wlast);
                     else solid_scan_round<false, true>(pos, opos, j0, 0u, xi, x0i, r2_out, ent, w, wlast);
                 }
                 if (j0 < e) {
-                    const unsigned ent = (r << kRowBits) | (j0 - b);
+                    const unsigned ent = ebase + (j0 - b);
                     if (__all(w <= wroom))
                         solid_scan_round<true, false>(pos, opos, j0, e - j0, xi, x0i, r2_out, ent, w, wlast);
                     else solid_scan_round<true, true>(pos, opos, j0, e - j0, xi, x0i, r2_out, ent, w, wlast);
@@ -437,7 +428,9 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                     js[q] = i;
                     if (k0 + q < cnt) {
                         ents[q] = col[(k0 + q) * kListBlock];
-                        js[q] = rows.pick(ents[q] >> kRowBits) + (ents[q] & ((1u << kSolidOffsetBits) - 1u));
+                        const unsigned gq = ents[q] >> kGroupBits;
+                        js[q] = (gq == 0u ? b0 : (gq == 1u ? b1 : b2)) +
+                                (ents[q] & ((1u << kSolidOffsetBits) - 1u));
                     }
                 }
                 V3 xs[kPairGroup];
@@ -446,14 +439,14 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
 #pragma unroll
                 for (unsigned q = 0; q < kPairGroup; ++q) {
                     if (k0 + q >= cnt) break;
-                    const bool cur = (ents[q] >> 11) & 1u;
+                    const bool cur = (ents[q] >> 13) & 1u;
                     if (cur) {
                         const float rr = length(xs[q] - xi);
                         acc.add(w_poly6(rr, kc));
                         if (rr < kc.h) ++nn;
                     }
                     c.list[static_cast<size_t>(k0 + q) * c.stride + i] =
-                        js[q] | (cur ? kListCurrent : 0u) | (((ents[q] >> 10) & 1u) ? kListReference : 0u);
+                        js[q] | (cur ? kListCurrent : 0u) | (((ents[q] >> 12) & 1u) ? kListReference : 0u);
                 }
             }
         } else {
