@@ -239,6 +239,14 @@ int sph_hip_get_fluid_pair_group(unsigned *entries);
 /* Name, architecture and compute-unit count of the selected device (the key
  * of the autotuner's table). */
 int sph_hip_device_name(char *buf, unsigned len);
+/* No counterpart in the reference: a self-check of the device code on the
+ * selected device.  The pair passes divide by a float through a double
+ * reciprocal formed from the hardware estimate and two Newton steps; this
+ * compares that reciprocal with the correctly rounded 1.0 / double(b) for
+ * every one of the 2^32 float bit patterns b and stores the number of
+ * patterns whose results differ in bits (NaN against NaN counts as equal)
+ * in *mismatches.  Takes a few milliseconds. */
+int sph_hip_check_recip(unsigned long long *mismatches);
 
 /* ---- the reference's device API under its own names ----------------------
  * src/opencl/particle_system_host.h:7-80 (and set_body_position/rotation,

@@ -28,6 +28,7 @@ SOURCES = [
     "runtime.hip",
     "batch.hip",
     "sums_noalloc.hip",
+    "selfcheck.hip",
     "opencl_api.cpp",
 ]
 

@@ -63,9 +63,25 @@ __device__ __forceinline__ V3 operator*(V3 a, float s) { return V3{a.x * s, a.y 
 // significands is never a float midpoint and lies at least 2^-49 (relative)
 // from every midpoint, in the normal and the subnormal range; the double
 // product is within 2^-52 of the quotient, so rounding it to float cannot
-// cross a midpoint.  Several quotients by one divisor then cost one double
-// division and a multiply each instead of one float division each.
-__device__ __forceinline__ double recip(float b) { return 1.0 / static_cast<double>(b); }
+// cross a midpoint.  Several quotients by one divisor then cost one
+// reciprocal and a multiply each instead of one float division each.
+//
+// The reciprocal is 1.0 / double(b) correctly rounded, formed without the
+// division sequence: the hardware estimate (relative error below 2^-24)
+// refined by two Newton steps y' = y + y (1 - b y) in fused multiply-adds.
+// For a float divisor this equals the correctly rounded quotient bit for bit:
+// sph_hip_check_recip compares the two for all 2^32 bit patterns on the
+// device.  Zeros, infinities and NaN make a Newton step NaN, and for them
+// the estimate itself is already exact (+-inf, +-0, NaN).
+__device__ __forceinline__ double recip(float b) {
+    const double bd = static_cast<double>(b);
+    const double y0 = __builtin_amdgcn_rcp(bd);
+    const double y1 = __builtin_fma(y0, __builtin_fma(-bd, y0, 1.0), y0);
+    const double y2 = __builtin_fma(y1, __builtin_fma(-bd, y1, 1.0), y1);
+    return __builtin_isfinite(y2) ? y2 : y0;
+}
+// The same value through the compiler's division, for the self-check.
+__device__ __forceinline__ double recip_by_division(float b) { return 1.0 / static_cast<double>(b); }
 __device__ __forceinline__ float div_recip(float a, double rb) {
     return static_cast<float>(static_cast<double>(a) * rb);
 }

@@ -143,6 +143,7 @@ _get_tuning = _fn("sph_hip_get_tuning", [POINTER(c_uint)], c_int)
 _set_pair_group = _fn("sph_hip_set_fluid_pair_group", [c_uint], c_int)
 _get_pair_group = _fn("sph_hip_get_fluid_pair_group", [POINTER(c_uint)], c_int)
 _device_name = _fn("sph_hip_device_name", [ctypes.c_char_p, c_uint], c_int)
+_check_recip = _fn("sph_hip_check_recip", [POINTER(c_ulonglong)], c_int)
 
 
 def init_hip():
@@ -168,6 +169,15 @@ def device_name():
     buf = ctypes.create_string_buffer(256)
     check(_device_name(buf, 256))
     return buf.value.decode()
+
+
+def check_recip():
+    """Number of float bit patterns whose device reciprocal (hardware
+    estimate + two Newton steps) differs from the correctly rounded
+    1.0 / double(b): 0 means the pair passes' divisions are exact."""
+    bad = c_ulonglong(0)
+    check(_check_recip(ctypes.byref(bad)))
+    return int(bad.value)
 
 
 def terminate_hip():
