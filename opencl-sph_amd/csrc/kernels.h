@@ -203,6 +203,8 @@ struct BatchTable {
     const unsigned *chunk_end;
     unsigned map;                 // chunk map index into layout->nchunks
     unsigned grid_units;          // host: slots (no chunk map) or chunks the launch covers
+    unsigned xcd_segment;         // chunks per segment dealt round-robin to the XCDs; 0: one
+                                  // contiguous eighth of the map per XCD (SceneTable::slot_xcd)
 };
 
 hipError_t launch_compute_density(const StepParams &p, const FieldPtrs &f, hipStream_t s);

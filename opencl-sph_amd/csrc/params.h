@@ -67,13 +67,23 @@ struct SceneTable {
         return i < t.chunk_end[k] ? i : kNoScene;
     }
     // XCD x takes the contiguous eighth [x * per, (x + 1) * per) of the
-    // chunks the layout holds, whatever the grid's size.
+    // chunks the layout holds, whatever the grid's size: neighbouring chunks
+    // share neighbour rows in one L2.  With a segment size that divides the
+    // grid's L chunk positions per XCD, the map is instead cut into segments
+    // of xcd_segment chunks dealt round-robin to the XCDs (segment q to XCD
+    // q % 8), covering chunks [0, 8 L) once: each XCD then meets the same mix
+    // of scenes, which balances passes whose cost per chunk varies with the
+    // scene.
     __device__ __forceinline__ unsigned slot_xcd(unsigned block_threads) const {
         if (!t.chunk_slot) return slot_linear(block_threads);
         const unsigned cpb = block_threads / kChunk;
-        const unsigned per = (t.layout->nchunks[t.map] + kXcds - 1u) / kXcds;
         const unsigned local = (blockIdx.x / kXcds) * cpb + threadIdx.x / kChunk;
-        return local < per ? chunk_lane((blockIdx.x % kXcds) * per + local) : kNoScene;
+        const unsigned x = blockIdx.x % kXcds;
+        const unsigned L = (gridDim.x / kXcds) * cpb;
+        const unsigned seg = t.xcd_segment;
+        if (seg != 0u && L % seg == 0u) return chunk_lane(((local / seg) * kXcds + x) * seg + local % seg);
+        const unsigned per = (t.layout->nchunks[t.map] + kXcds - 1u) / kXcds;
+        return local < per ? chunk_lane(x * per + local) : kNoScene;
     }
     __device__ __forceinline__ unsigned slot_linear(unsigned block_threads) const {
         if (!t.chunk_slot) return blockIdx.x * block_threads + threadIdx.x;
