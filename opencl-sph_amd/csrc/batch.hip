@@ -348,19 +348,46 @@ sphk::BatchTable table(const sph_batch *b, int map) {
 
 // Uploads what the passes read of the layout, when it changed: the
 // segments, the per-system chunk maps (the 64-slot chunks of the scenes with
-// steps left, in slot order) and the counts.  Enqueued on the batch stream,
-// so that it takes effect after the steps enqueued before it.
+// steps left) and the counts.  Enqueued on the batch stream, so that it
+// takes effect after the steps enqueued before it.
+//
+// The solid maps list their scenes in slot order.  The fluid map lists
+// whole scenes grouped into kXcds bins of nearly equal chunk counts (the
+// largest scene first, each into the bin with the fewest chunks so far),
+// then bin after bin: with its passes dealing 64-chunk segments to the XCDs
+// this spreads large and small scenes (a chunk's cost follows its scene's
+// size and surface) more evenly than slot order.  Measured on the served
+// stream's profiled pass: fluid density 274 -> 235 us, fluid forces 245 ->
+// 209 us per launch; the solid passes were 1-2 % slower with the binned
+// order and keep slot order.
 int upload_layout(sph_batch *b) {
     if (b->uploaded_version == b->layout_version) return 0;
     std::vector<unsigned> slot[sph_batch::kMaps], end[sph_batch::kMaps];
+    std::vector<const StepParams *> members[sph_batch::kMaps];
     for (const Segment &g : b->segs) {
         if (g.scene == kNoScene || b->steps_left[g.scene] == 0) continue;
         const StepParams &p = b->scenes[g.scene];
         const int m = p.system == sphk::kFluid ? kMapFluid : p.system == sphk::kCorotational ? kMapCorot : kMapSvk;
-        for (unsigned c = 0; c < p.n; c += sphk::kChunk) {
-            slot[m].push_back(p.p0 + c);
-            end[m].push_back(p.p0 + p.n);
+        members[m].push_back(&p);
+    }
+    for (int m = 0; m < sph_batch::kMaps; ++m) {
+        std::vector<const StepParams *> &ms = members[m];
+        if (m == kMapFluid)
+            std::stable_sort(ms.begin(), ms.end(), [](const StepParams *a, const StepParams *c) { return a->n > c->n; });
+        std::vector<const StepParams *> bins[sphk::kXcds];
+        size_t load[sphk::kXcds] = {};
+        for (const StepParams *p : ms) {
+            const size_t x =
+                m == kMapFluid ? static_cast<size_t>(std::min_element(load, load + sphk::kXcds) - load) : 0u;
+            bins[x].push_back(p);
+            load[x] += (p->n + sphk::kChunk - 1u) / sphk::kChunk;
         }
+        for (const auto &bin : bins)
+            for (const StepParams *p : bin)
+                for (unsigned c = 0; c < p->n; c += sphk::kChunk) {
+                    slot[m].push_back(p->p0 + c);
+                    end[m].push_back(p->p0 + p->n);
+                }
     }
     sphk::BatchLayout lay{};
     lay.used_slots = b->used_slots;
