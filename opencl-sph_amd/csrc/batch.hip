@@ -332,11 +332,11 @@ unsigned chunk_class(const sph_batch *b) {
     return size_class(most, 64u, static_cast<unsigned>(b->max_chunks));
 }
 
-// The scene table restricted to the chunks of one system's running scenes.
 // Chunks per XCD segment of the passes that balance their chunk map over
 // the XCDs (SceneTable::slot_xcd).
 constexpr unsigned kBalancedSegment = 64;
 
+// The scene table restricted to the chunks of one system's running scenes.
 sphk::BatchTable table(const sph_batch *b, int map) {
     sphk::BatchTable t = table(b);
     t.chunk_slot = b->d_chunk_slot[map];
