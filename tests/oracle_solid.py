@@ -105,6 +105,30 @@ class OracleSolid:
         st.step(self.system, 1)
         self._store(st, names, n)
 
+    def set_body_position(self, body, position):
+        """set_body_position_device_opencl on the host fields
+        (src/opencl/particle_system_host.c:562-575, kernels/rigidbody.cl)."""
+        g = lambda name: self.get_fields(name)[0]
+        n = int(g("n")[0])
+        pos, vel, bpos = oracle.set_body_position(
+            g("position")[:n], g("velocity")[:n], g("body_number")[:n], n, int(g("body_count")[0]),
+            float(g("timestep")[0]), int(body), g("body_position"), np.asarray(position, np.float32))
+        g("position")[:n] = pos
+        g("velocity")[:n] = vel
+        g("body_position")[...] = bpos.reshape(g("body_position").shape)
+
+    def set_body_rotation(self, body, euler_angles):
+        """set_body_rotation_device_opencl on the host fields
+        (src/opencl/particle_system_host.c:576-588, kernels/rigidbody.cl)."""
+        g = lambda name: self.get_fields(name)[0]
+        n = int(g("n")[0])
+        pos, rot, beul = oracle.set_body_rotation(
+            g("position")[:n], g("rotation")[:n], g("body_number")[:n], n, int(g("body_count")[0]), int(body),
+            g("body_euler_rotation"), np.asarray(euler_angles, np.float32))
+        g("position")[:n] = pos
+        g("rotation")[:n] = rot.reshape(g("rotation")[:n].shape)
+        g("body_euler_rotation")[...] = beul.reshape(g("body_euler_rotation").shape)
+
     def call_for_all_particles(self, kernel):
         if kernel != "apply_plane_constraints":
             raise NotImplementedError(kernel)
