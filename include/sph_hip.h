@@ -247,6 +247,11 @@ int sph_hip_device_name(char *buf, unsigned len);
  * patterns whose results differ in bits (NaN against NaN counts as equal)
  * in *mismatches.  Takes a few milliseconds. */
 int sph_hip_check_recip(unsigned long long *mismatches);
+/* No counterpart in the reference: the same sweep for the square root of the
+ * fluid density pass (the hardware estimate corrected to the rounded root by
+ * fused multiply-add residuals), against the compiler's correctly rounded
+ * sqrtf, over every float bit pattern. */
+int sph_hip_check_sqrt(unsigned long long *mismatches);
 
 /* ---- the reference's device API under its own names ----------------------
  * src/opencl/particle_system_host.h:7-80 (and set_body_position/rotation,

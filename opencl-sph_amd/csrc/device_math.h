@@ -93,6 +93,37 @@ __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y 
 __device__ __forceinline__ float sqlength(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
 __device__ __forceinline__ float length(V3 a) { return sqrtf(sqlength(a)); }
 
+// Range prefilter.  sqlength_fused rounds twice instead of five times; with
+// non-negative terms each of the two forms lies within a factor (1 +- u)^3 of
+// the exact a.a (u = 2^-24), so sqlength(a) < r2 implies
+// sqlength_fused(a) < r2 (1 + 6.1 u) <= widened_r2(r2).  NaN components make
+// both forms NaN.  A scan that appends on the prefilter re-tests each hit
+// with sqlength, so the set it keeps is the one sqlength selects.
+__device__ __forceinline__ float sqlength_fused(V3 a) {
+    return __builtin_fmaf(a.z, a.z, __builtin_fmaf(a.y, a.y, a.x * a.x));
+}
+__device__ __forceinline__ float widened_r2(float r2) { return r2 * (1.0f + 0x1p-20f); }
+
+// Correctly rounded square root (the value of sqrtf) without the small-
+// argument scaling of the compiler's sequence: the hardware estimate r, then
+// the fused multiply-add residuals s - r'r of its one-ulp neighbours r' pick
+// the rounded root.  That holds for s >= 2^-96 and for +-0 and +inf; other
+// arguments (smaller positive s, negative s, NaN) take sqrtf, behind a
+// wave-uniform branch that the pair passes practically never take.
+__device__ __forceinline__ float sqrt_rn(float s) {
+    const float r = __builtin_amdgcn_sqrtf(s);
+    const float rm = __uint_as_float(__float_as_uint(r) - 1u);
+    const float rp = __uint_as_float(__float_as_uint(r) + 1u);
+    float out = __builtin_fmaf(-rm, r, s) <= 0.0f ? rm : r;
+    out = __builtin_fmaf(-rp, r, s) > 0.0f ? rp : out;
+    const bool other = !(s >= 0x1p-96f) && s != 0.0f;
+    if (__builtin_expect(__any(other), 0)) {
+        const float t = sqrtf(s);
+        out = other ? t : out;
+    }
+    return out;
+}
+
 // Reflection of point `a` in the plane through `pt` with normal `nrm`:
 // a - 2 n (n . (a - pt))   (kernels/common.cl:672).
 __device__ __forceinline__ V3 reflect_point(V3 a, V3 pt, V3 nrm) {
@@ -114,6 +145,11 @@ struct KernelConsts {
 // W_poly6(r), zero beyond h   (kernels/common.cl:78-80)
 __device__ __forceinline__ float w_poly6(float r, const KernelConsts &k) {
     if (r > k.h) return 0.0f;
+    float t = k.h2 - r * r;
+    return k.poly6 * (t * t * t);
+}
+// The same for r known to be <= h (a listed pair).
+__device__ __forceinline__ float w_poly6_in_range(float r, const KernelConsts &k) {
     float t = k.h2 - r * r;
     return k.poly6 * (t * t * t);
 }

@@ -140,7 +140,9 @@ hipError_t launch_scatter_words(const void *src, void *dst, const unsigned *map,
 // Neighbour lists written by the density pass, entries k < count[i] at
 // list[k * stride + i] in the reference's visiting order (self included);
 // count[i] > kListCap means the list overflowed (or was not built for slot
-// i) and readers scan the candidates instead.
+// i) and readers scan the candidates instead.  A fluid list may also hold
+// extra entries of slot i itself, which the force pass skips as it skips
+// j == i.
 //   kFluidList: the slots j with length(x_j - x_i) <= h, as 16-bit entries
 //     (see kFluidPlanes).
 //   kSolidList: the slots j with length(x_j - x_i) <= h (flag

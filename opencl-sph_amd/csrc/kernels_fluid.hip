@@ -116,14 +116,17 @@ __global__ void __launch_bounds__(kBlock) k_density(PS ps, FieldPtrs f) {
 }
 
 // compute_density for fluid programs, in two phases per lane:
-//   1. scan the candidates with the square-root-free range test and compact
-//      the in-range slots into this lane's column of an LDS buffer, as 16-bit
-//      entries (dz group << 14 | offset from the group's first slot);
-//   2. evaluate the kernel sum over the compacted slots (every lane busy),
-//      writing them out as the pair cache's neighbour list.
+//   1. scan the candidates with the range prefilter (sqlength_fused, a
+//      superset of the square-root-free test) and compact the hits into this
+//      lane's column of an LDS buffer, as 16-bit entries (dz group << 14 |
+//      offset from the group's first slot);
+//   2. re-test each hit with sqlength and evaluate the kernel sum over those
+//      in range (every lane busy), writing the entries out as the pair
+//      cache's neighbour list; a hit out of range is written as an entry of
+//      the slot itself, which the force pass skips as it skips j == i.
 // The summation order is the candidate order, as in the reference.  Lanes
-// near a mirror plane, with more than kListCap in-range candidates or with a
-// dz group spanning more than 16384 slots take the reference loop instead.
+// near a mirror plane, with more than kListCap hits or with a dz group
+// spanning more than 16384 slots take the reference loop instead.
 constexpr unsigned kListBlock = 128;
 
 // First slot of neighbour row r (0..8, (dz, dy) = (r / 3 - 1, r % 3 - 1))
@@ -146,14 +149,14 @@ __device__ __forceinline__ bool row_range(const GridGeom &g, unsigned lo, unsign
 
 // One round of the fluid scan: kScanUnroll candidates from slot j0, loads
 // first (through a 64-bit address, so that the round's offsets fold into the
-// loads), then the range tests in slot order, appending ent + u to the
-// lane's LDS column at w for each hit.  CHECK_LEFT: the round reaches past
+// loads), then the range prefilter (sqlength_fused against r2_wide) in slot
+// order, appending ent + u to the lane's LDS column at w for each hit.  CHECK_LEFT: the round reaches past
 // the row's end, and only its first `left` candidates are used.  CLAMP: the
 // column may fill up during the round; past its last entry, that entry is
 // overwritten (the list is then discarded, its count being past kListCap).
 template <bool CHECK_LEFT, bool CLAMP>
 __device__ __forceinline__ void scan_round(const float *__restrict__ pos, unsigned j0, unsigned left, V3 xi,
-                                           float r2_out, unsigned ent, unsigned short *&w,
+                                           float r2_wide, unsigned ent, unsigned short *&w,
                                            unsigned short *wlast) {
     const Packed3 *q = reinterpret_cast<const Packed3 *>(pos) + j0;
     V3 xs[kScanUnroll];
@@ -164,8 +167,8 @@ __device__ __forceinline__ void scan_round(const float *__restrict__ pos, unsign
     }
 #pragma unroll
     for (unsigned u = 0; u < kScanUnroll; ++u) {
-        const float s = sqlength(xs[u] - xi);
-        const bool hit = CHECK_LEFT ? ((u < left) & !(s >= r2_out)) : !(s >= r2_out);
+        const float s = sqlength_fused(xs[u] - xi);
+        const bool hit = CHECK_LEFT ? ((u < left) & !(s >= r2_wide)) : !(s >= r2_wide);
         if (hit) {
             *(CLAMP ? min(w, wlast) : w) = static_cast<unsigned short>(ent + u);
             w += kListBlock;
@@ -197,6 +200,7 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     const float *__restrict__ pos = f.position;
     const unsigned lo = p.p0, hi = p.p0 + p.n;
     const float r2_out = p.r2_out;
+    const float r2_wide = widened_r2(r2_out);
     const GridGeom g = p.grid;
     const V3 xi = load3(pos, i);
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
@@ -243,13 +247,13 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
             unsigned j0 = b;
             for (; j0 + kScanUnroll <= e; j0 += kScanUnroll) {
                 const unsigned ent = ebase + (j0 - b);
-                if (__all(w <= wroom)) scan_round<false, false>(pos, j0, 0u, xi, r2_out, ent, w, wlast);
-                else scan_round<false, true>(pos, j0, 0u, xi, r2_out, ent, w, wlast);
+                if (__all(w <= wroom)) scan_round<false, false>(pos, j0, 0u, xi, r2_wide, ent, w, wlast);
+                else scan_round<false, true>(pos, j0, 0u, xi, r2_wide, ent, w, wlast);
             }
             if (j0 < e) {
                 const unsigned ent = ebase + (j0 - b);
-                if (__all(w <= wroom)) scan_round<true, false>(pos, j0, e - j0, xi, r2_out, ent, w, wlast);
-                else scan_round<true, true>(pos, j0, e - j0, xi, r2_out, ent, w, wlast);
+                if (__all(w <= wroom)) scan_round<true, false>(pos, j0, e - j0, xi, r2_wide, ent, w, wlast);
+                else scan_round<true, true>(pos, j0, e - j0, xi, r2_wide, ent, w, wlast);
             }
         }
         cnt = static_cast<unsigned>(w - col) / kListBlock;
@@ -262,6 +266,9 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
         // slot's own column (see kFluidPlanes), with the group bases
         static_assert(kPairGroup % 2 == 0, "entries are stored in pairs");
         Acc<KAHAN> acc;
+        // the slot's own entry: its cell is in the dz = 0 group, whose span
+        // the scan checked
+        const unsigned self_ent = (1u << kGroupBits) | (i - gb.b1);
         for (unsigned k0 = 0; k0 < cnt; k0 += kPairGroup) {
             unsigned es[kPairGroup], js[kPairGroup];
 #pragma unroll
@@ -279,9 +286,15 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
 #pragma unroll
             for (unsigned q = 0; q < kPairGroup; ++q) {
                 if (k0 + q >= cnt) break;
-                const float rr = length(xs[q] - xi);
-                acc.add(w_poly6(rr, kc));
-                if (rr < kc.h) ++nn;
+                const float sq = sqlength(xs[q] - xi);
+                if (!(sq >= r2_out)) {
+                    // in range: RN(sqrt(sq)) <= h
+                    const float rr = sqrt_rn(sq);
+                    acc.add(w_poly6_in_range(rr, kc));
+                    nn += rr < kc.h ? 1u : 0u;
+                } else {
+                    es[q] = self_ent;
+                }
             }
 #pragma unroll
             for (unsigned q = 0; q < kPairGroup; q += 2)
@@ -590,8 +603,9 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
     FluidForce<KAHAN> acc;
     const unsigned cnt = c.count[i];
     if (cnt <= kListCap) {
-        // List entries are in range by construction (same positions, same
-        // distance expression), so the pair terms need no range test here.
+        // List entries other than slot i are in range by construction (same
+        // positions, same distance expression), so the pair terms need no
+        // range test here.
         struct Nbr {
             float4 a, b;
         };

@@ -144,6 +144,7 @@ _set_pair_group = _fn("sph_hip_set_fluid_pair_group", [c_uint], c_int)
 _get_pair_group = _fn("sph_hip_get_fluid_pair_group", [POINTER(c_uint)], c_int)
 _device_name = _fn("sph_hip_device_name", [ctypes.c_char_p, c_uint], c_int)
 _check_recip = _fn("sph_hip_check_recip", [POINTER(c_ulonglong)], c_int)
+_check_sqrt = _fn("sph_hip_check_sqrt", [POINTER(c_ulonglong)], c_int)
 
 
 def init_hip():
@@ -177,6 +178,15 @@ def check_recip():
     1.0 / double(b): 0 means the pair passes' divisions are exact."""
     bad = c_ulonglong(0)
     check(_check_recip(ctypes.byref(bad)))
+    return int(bad.value)
+
+
+def check_sqrt():
+    """Number of float bit patterns whose device square root in the fluid
+    density pass (hardware estimate + residual correction) differs from the
+    correctly rounded sqrtf: 0 means the pass's distances are exact."""
+    bad = c_ulonglong(0)
+    check(_check_sqrt(ctypes.byref(bad)))
     return int(bad.value)
 
 
