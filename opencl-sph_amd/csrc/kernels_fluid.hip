@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(kBlock) k_density(PS ps, FieldPtrs f) {
 //      cache's neighbour list; a hit out of range is written as an entry of
 //      the slot itself, which the force pass skips as it skips j == i.
 // The summation order is the candidate order, as in the reference.  Lanes
-// near a mirror plane, with more than kListCap hits or with a dz group
+// near a mirror plane, with more than kFluidListCap hits or with a dz group
 // spanning more than 16384 slots take the reference loop instead.
 constexpr unsigned kListBlock = 128;
 
@@ -147,24 +147,15 @@ __device__ __forceinline__ bool row_range(const GridGeom &g, unsigned lo, unsign
     return true;
 }
 
-// One round of the fluid scan: kScanUnroll candidates from slot j0, loads
-// first (through a 64-bit address, so that the round's offsets fold into the
-// loads), then the range prefilter (sqlength_fused against r2_wide) in slot
-// order, appending ent + u to the lane's LDS column at w for each hit.  CHECK_LEFT: the round reaches past
-// the row's end, and only its first `left` candidates are used.  CLAMP: the
-// column may fill up during the round; past its last entry, that entry is
+// One round of the fluid scan: the range prefilter (sqlength_fused against
+// r2_wide) on kScanUnroll candidates xs in slot order, appending ent + u to
+// the lane's LDS column at w for each hit.  CHECK_LEFT: the round reaches
+// past the row's end, and only its first `left` candidates are used.  CLAMP:
+// the column may fill up during the round; past its last entry, that entry is
 // overwritten (the list is then discarded, its count being past kListCap).
 template <bool CHECK_LEFT, bool CLAMP>
-__device__ __forceinline__ void scan_round(const float *__restrict__ pos, unsigned j0, unsigned left, V3 xi,
-                                           float r2_wide, unsigned ent, unsigned short *&w,
-                                           unsigned short *wlast) {
-    const Packed3 *q = reinterpret_cast<const Packed3 *>(pos) + j0;
-    V3 xs[kScanUnroll];
-#pragma unroll
-    for (unsigned u = 0; u < kScanUnroll; ++u) {
-        const Packed3 t = q[u];
-        xs[u] = V3{t.x, t.y, t.z};
-    }
+__device__ __forceinline__ void scan_round(const V3 (&xs)[kScanUnroll], unsigned left, V3 xi, float r2_wide,
+                                           unsigned ent, unsigned short *&w, unsigned short *wlast) {
 #pragma unroll
     for (unsigned u = 0; u < kScanUnroll; ++u) {
         const float s = sqlength_fused(xs[u] - xi);
@@ -174,6 +165,37 @@ __device__ __forceinline__ void scan_round(const float *__restrict__ pos, unsign
             w += kListBlock;
         }
     }
+}
+// The candidates of a round: from global memory (the round's offsets fold
+// into the loads), or from the wave's LDS copy of the row (below).
+template <unsigned N>
+__device__ __forceinline__ void candidates_from(const Packed3 *q, V3 (&xs)[N]) {
+#pragma unroll
+    for (unsigned u = 0; u < N; ++u) {
+        const Packed3 t = q[u];
+        xs[u] = V3{t.x, t.y, t.z};
+    }
+}
+
+// Row staging.  The lanes of a wave hold consecutive slots of one scene, so
+// their cells are in key order and so are the slot ranges of each of their
+// neighbour rows: the union of a row's ranges over the wave is the
+// consecutive slots from its first scanning lane's start to its last
+// scanning lane's end.  The wave copies those positions into LDS once and
+// its lanes read their candidates there, instead of every lane reading its
+// 20-30 candidates through the vector cache.  A row whose union exceeds
+// kStageCap slots is scanned from global memory.  The LDS of a block (the
+// lanes' list columns of kFluidListCap entries and the staged rows) is sized
+// so that LDS does not limit the resident waves below what the registers
+// allow.
+constexpr unsigned kWave = 64;
+constexpr unsigned kStageCap = 96;
+constexpr unsigned kFluidListCap = 40;
+static_assert(kFluidListCap <= kListCap, "a fluid list fits the pair cache's columns");
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // First slots of the three dz groups of neighbour rows of a lane (rows 3g
@@ -192,7 +214,9 @@ struct GroupBases {
 
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs f, PairCache c) {
-    __shared__ unsigned short buf[kListCap * kListBlock];
+    __shared__ unsigned short buf[kFluidListCap * kListBlock];
+    // per wave: one staged row, with room for a round read past its end
+    __shared__ Packed3 stage[kListBlock / kWave][kStageCap + kScanUnroll];
     const unsigned i = ps.slot_xcd(kListBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
@@ -206,11 +230,21 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned key = f.gridcell[i];
     unsigned short *col = buf + threadIdx.x;
+    Packed3 *const stg = stage[threadIdx.x / kWave];
     unsigned cnt = 0;
-    bool fast = !inear && key >= g.c0 && key - g.c0 < g.ncells;
+    const bool key_in_grid = key >= g.c0 && key - g.c0 < g.ncells;
+    bool fast = !inear && key_in_grid;
     GroupBases gb;
-    if (fast) {
-        const unsigned local = key - g.c0;
+    // Every active lane of the wave takes part in the row loop (to stage
+    // rows); the lanes on the fast path scan.  The copy is spread over the
+    // active lanes by their rank among them (slots past a scene's end have
+    // left the kernel).
+    if (__any(fast)) {
+        const unsigned long long act = __ballot(1);
+        const unsigned copier = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(act >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(act), 0u));
+        const unsigned copiers = static_cast<unsigned>(__builtin_popcountll(act));
+        const unsigned local = key_in_grid ? key - g.c0 : 0u;
         const unsigned zo = local / g.layer;
         const unsigned rem = local - g.layer * zo;
         const unsigned yo = rem / g.rx;
@@ -218,8 +252,8 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
         // next free entry of the lane's column; the column's last entry; the
         // last position with room for a whole round
         unsigned short *w = col;
-        unsigned short *const wlast = col + (kListCap - 1u) * kListBlock;
-        unsigned short *const wroom = col + (kListCap - kScanUnroll) * kListBlock;
+        unsigned short *const wlast = col + (kFluidListCap - 1u) * kListBlock;
+        unsigned short *const wroom = col + (kFluidListCap - kScanUnroll) * kListBlock;
         unsigned gbase = 0;   // first slot of the current group's first row inside the grid
         bool gset = false;
 #pragma unroll 1
@@ -227,37 +261,60 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
             const unsigned gi = r / 3u;
             if (r == 3u * gi) gset = false;
             unsigned b = 0, e = 0;
-            const bool inside = row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e);
-            if (!inside) continue;
-            if (!gset) {
-                gbase = b;
-                gset = true;
-                if (gi == 0u) gb.b0 = b;
-                else if (gi == 1u) gb.b1 = b;
-                else gb.b2 = b;
+            const bool scan = row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e) && fast;
+            const unsigned long long m = __ballot(scan);
+            if (m == 0) continue;
+            // the wave's union of this row's ranges: [first lane's b, last lane's e)
+            const unsigned sb = __builtin_amdgcn_readlane(b, __builtin_ctzll(m));
+            const unsigned se = __builtin_amdgcn_readlane(e, 63 - __builtin_clzll(m));
+            const bool staged = se - sb <= kStageCap;
+            if (staged) {
+                for (unsigned k = copier; k < se - sb; k += copiers) {
+                    const V3 x = load3(pos, sb + k);
+                    stg[k] = Packed3{x.x, x.y, x.z};
+                }
+                wave_sync();
             }
-            if (e - gbase > (1u << kGroupBits)) fast = false;
-            // kScanUnroll candidates per round, loads first (slots past the
-            // row's end are read, within the buffer's kScanPad padding, and
-            // not used), tested in slot order: entry ebase + (j0 - b) + u.
-            static_assert(kScanUnroll <= kScanPad, "scan rounds read past the row end");
-            const unsigned ebase = (gi << kGroupBits) + (b - gbase);
-            // Whole rounds need no end-of-row test; a round needs no cap test
-            // when every lane of the wave has room for all its candidates.
-            unsigned j0 = b;
-            for (; j0 + kScanUnroll <= e; j0 += kScanUnroll) {
-                const unsigned ent = ebase + (j0 - b);
-                if (__all(w <= wroom)) scan_round<false, false>(pos, j0, 0u, xi, r2_wide, ent, w, wlast);
-                else scan_round<false, true>(pos, j0, 0u, xi, r2_wide, ent, w, wlast);
+            if (scan) {
+                if (!gset) {
+                    gbase = b;
+                    gset = true;
+                    if (gi == 0u) gb.b0 = b;
+                    else if (gi == 1u) gb.b1 = b;
+                    else gb.b2 = b;
+                }
+                if (e - gbase > (1u << kGroupBits)) fast = false;
+                // kScanUnroll candidates per round, tested in slot order:
+                // entry ebase + (j0 - b) + u.  The last round of a row reads
+                // past its end (within the buffer's kScanPad padding, or the
+                // stage's spare room) and does not use those candidates.
+                static_assert(kScanUnroll <= kScanPad, "scan rounds read past the row end");
+                const unsigned ebase = (gi << kGroupBits) + (b - gbase);
+                // Whole rounds need no end-of-row test; a round needs no cap
+                // test when every lane of the wave has room for all its
+                // candidates.
+                unsigned j0 = b;
+                V3 xs[kScanUnroll];
+                for (; j0 + kScanUnroll <= e; j0 += kScanUnroll) {
+                    const unsigned ent = ebase + (j0 - b);
+                    if (staged) candidates_from(stg + (j0 - sb), xs);
+                    else candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
+                    if (__all(w <= wroom)) scan_round<false, false>(xs, 0u, xi, r2_wide, ent, w, wlast);
+                    else scan_round<false, true>(xs, 0u, xi, r2_wide, ent, w, wlast);
+                }
+                if (j0 < e) {
+                    const unsigned ent = ebase + (j0 - b);
+                    if (staged) candidates_from(stg + (j0 - sb), xs);
+                    else candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
+                    if (__all(w <= wroom)) scan_round<true, false>(xs, e - j0, xi, r2_wide, ent, w, wlast);
+                    else scan_round<true, true>(xs, e - j0, xi, r2_wide, ent, w, wlast);
+                }
             }
-            if (j0 < e) {
-                const unsigned ent = ebase + (j0 - b);
-                if (__all(w <= wroom)) scan_round<true, false>(pos, j0, e - j0, xi, r2_wide, ent, w, wlast);
-                else scan_round<true, true>(pos, j0, e - j0, xi, r2_wide, ent, w, wlast);
-            }
+            // this row's reads before the next row's copy
+            if (staged) wave_sync();
         }
         cnt = static_cast<unsigned>(w - col) / kListBlock;
-        if (cnt > kListCap) fast = false;
+        if (cnt > kFluidListCap) fast = false;
     }
     float rho;
     unsigned nn = 0;
@@ -330,23 +387,15 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
 // without LDS.
 constexpr unsigned kSolidOffsetBits = 12;
 
-// One round of the solid scan: kSolidScanUnroll candidates from slot j0,
-// tested against the current and the reference position, appending
+// One round of the solid scan: kSolidScanUnroll candidates (current
+// positions xs, reference positions x0s) tested against the current and the
+// reference position of the lane, appending
 // (ent + u) | current << 13 | reference << 12 for each candidate in either
 // range.  CHECK_LEFT and CLAMP as in scan_round.
 template <bool CHECK_LEFT, bool CLAMP>
-__device__ __forceinline__ void solid_scan_round(const float *__restrict__ pos, const float *__restrict__ opos,
-                                                 unsigned j0, unsigned left, V3 xi, V3 x0i, float r2_out,
-                                                 unsigned ent, unsigned short *&w, unsigned short *wlast) {
-    const Packed3 *q = reinterpret_cast<const Packed3 *>(pos) + j0;
-    const Packed3 *q0 = reinterpret_cast<const Packed3 *>(opos) + j0;
-    V3 xs[kSolidScanUnroll], x0s[kSolidScanUnroll];
-#pragma unroll
-    for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
-        const Packed3 t = q[u], t0 = q0[u];
-        xs[u] = V3{t.x, t.y, t.z};
-        x0s[u] = V3{t0.x, t0.y, t0.z};
-    }
+__device__ __forceinline__ void solid_scan_round(const V3 (&xs)[kSolidScanUnroll], const V3 (&x0s)[kSolidScanUnroll],
+                                                 unsigned left, V3 xi, V3 x0i, float r2_out, unsigned ent,
+                                                 unsigned short *&w, unsigned short *wlast) {
 #pragma unroll
     for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
         const bool cur = !(sqlength(xs[u] - xi) >= r2_out);
@@ -359,10 +408,18 @@ __device__ __forceinline__ void solid_scan_round(const float *__restrict__ pos, 
         }
     }
 }
+// The solid scan's LDS: list columns of kSolidListCap entries and per wave
+// the current and reference positions of one row of up to kSolidStageCap
+// slots (as for the fluid scan).
+constexpr unsigned kSolidListCap = kListCap;
+constexpr unsigned kSolidStageCap = 80;
 
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, FieldPtrs f, PairCache c) {
-    __shared__ unsigned short buf[kListCap * kListBlock];
+    __shared__ unsigned short buf[kSolidListCap * kListBlock];
+    // per wave: one staged row (current, reference), with room for a round
+    // read past its end
+    __shared__ Packed3 stage[kListBlock / kWave][2][kSolidStageCap + kSolidScanUnroll];
     const unsigned i = ps.slot_xcd(kListBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
@@ -376,33 +433,50 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned key = f.gridcell[i];
     unsigned short *col = buf + threadIdx.x;
+    Packed3 *const stg = stage[threadIdx.x / kWave][0];
+    Packed3 *const stg0 = stage[threadIdx.x / kWave][1];
+    const V3 x0i = load3(opos, i);
+    const bool key_in_grid = key >= g.c0 && key - g.c0 < g.ncells;
+    bool fast = !inear && key_in_grid;
     unsigned nn = 0, cnt = 0;
-    float rho;
-    if (inear) {
-        rho = density_scan<KAHAN, true>(p, f, i, xi, inear, &nn, nullptr, 0, nullptr);
-        cnt = kListCap + 1u;
-    } else {
-        const V3 x0i = load3(opos, i);
-        bool fast = key >= g.c0 && key - g.c0 < g.ncells;
-        unsigned b0 = 0, b1 = 0, b2 = 0;   // dz group bases, as in k_density_listed
-        if (fast) {
-            const unsigned local = key - g.c0;
-            const unsigned zo = local / g.layer;
-            const unsigned rem = local - g.layer * zo;
-            const unsigned yo = rem / g.rx;
-            const unsigned xo = rem - g.rx * yo;
-            unsigned short *w = col;   // as in k_density_listed
-            unsigned short *const wlast = col + (kListCap - 1u) * kListBlock;
-            unsigned short *const wroom = col + (kListCap - kSolidScanUnroll) * kListBlock;
-            unsigned gbase = 0;
-            bool gset = false;
+    unsigned b0 = 0, b1 = 0, b2 = 0;   // dz group bases, as in k_density_listed
+    // the row loop as in k_density_listed: every active lane stages, the
+    // lanes on the fast path scan
+    if (__any(fast)) {
+        const unsigned long long act = __ballot(1);
+        const unsigned copier = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(act >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(act), 0u));
+        const unsigned copiers = static_cast<unsigned>(__builtin_popcountll(act));
+        const unsigned local = key_in_grid ? key - g.c0 : 0u;
+        const unsigned zo = local / g.layer;
+        const unsigned rem = local - g.layer * zo;
+        const unsigned yo = rem / g.rx;
+        const unsigned xo = rem - g.rx * yo;
+        unsigned short *w = col;   // as in k_density_listed
+        unsigned short *const wlast = col + (kSolidListCap - 1u) * kListBlock;
+        unsigned short *const wroom = col + (kSolidListCap - kSolidScanUnroll) * kListBlock;
+        unsigned gbase = 0;
+        bool gset = false;
 #pragma unroll 1
-            for (unsigned r = 0; r < 9u; ++r) {
-                const unsigned gi = r / 3u;
-                if (r == 3u * gi) gset = false;
-                unsigned b = 0, e = 0;
-                const bool inside = row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e);
-                if (!inside) continue;
+        for (unsigned r = 0; r < 9u; ++r) {
+            const unsigned gi = r / 3u;
+            if (r == 3u * gi) gset = false;
+            unsigned b = 0, e = 0;
+            const bool scan = row_range(g, lo, hi, xo, yo, zo, r, f.celloffset, f.gridcount, &b, &e) && fast;
+            const unsigned long long m = __ballot(scan);
+            if (m == 0) continue;
+            const unsigned sb = __builtin_amdgcn_readlane(b, __builtin_ctzll(m));
+            const unsigned se = __builtin_amdgcn_readlane(e, 63 - __builtin_clzll(m));
+            const bool staged = se - sb <= kSolidStageCap;
+            if (staged) {
+                for (unsigned k = copier; k < se - sb; k += copiers) {
+                    const V3 x = load3(pos, sb + k), x0 = load3(opos, sb + k);
+                    stg[k] = Packed3{x.x, x.y, x.z};
+                    stg0[k] = Packed3{x0.x, x0.y, x0.z};
+                }
+                wave_sync();
+            }
+            if (scan) {
                 if (!gset) {
                     gbase = b;
                     gset = true;
@@ -415,22 +489,44 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                 // entry of slot j: (gi << kGroupBits) + (j - gbase), ent + u in a round from j0
                 const unsigned ebase = (gi << kGroupBits) + (b - gbase);
                 unsigned j0 = b;
+                V3 xs[kSolidScanUnroll], x0s[kSolidScanUnroll];
                 for (; j0 + kSolidScanUnroll <= e; j0 += kSolidScanUnroll) {
                     const unsigned ent = ebase + (j0 - b);
+                    if (staged) {
+                        candidates_from(stg + (j0 - sb), xs);
+                        candidates_from(stg0 + (j0 - sb), x0s);
+                    } else {
+                        candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
+                        candidates_from(reinterpret_cast<const Packed3 *>(opos) + j0, x0s);
+                    }
                     if (__all(w <= wroom))
-                        solid_scan_round<false, false>(pos, opos, j0, 0u, xi, x0i, r2_out, ent, w, wlast);
-                    else solid_scan_round<false, true>(pos, opos, j0, 0u, xi, x0i, r2_out, ent, w, wlast);
+                        solid_scan_round<false, false>(xs, x0s, 0u, xi, x0i, r2_out, ent, w, wlast);
+                    else solid_scan_round<false, true>(xs, x0s, 0u, xi, x0i, r2_out, ent, w, wlast);
                 }
                 if (j0 < e) {
                     const unsigned ent = ebase + (j0 - b);
+                    if (staged) {
+                        candidates_from(stg + (j0 - sb), xs);
+                        candidates_from(stg0 + (j0 - sb), x0s);
+                    } else {
+                        candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
+                        candidates_from(reinterpret_cast<const Packed3 *>(opos) + j0, x0s);
+                    }
                     if (__all(w <= wroom))
-                        solid_scan_round<true, false>(pos, opos, j0, e - j0, xi, x0i, r2_out, ent, w, wlast);
-                    else solid_scan_round<true, true>(pos, opos, j0, e - j0, xi, x0i, r2_out, ent, w, wlast);
+                        solid_scan_round<true, false>(xs, x0s, e - j0, xi, x0i, r2_out, ent, w, wlast);
+                    else solid_scan_round<true, true>(xs, x0s, e - j0, xi, x0i, r2_out, ent, w, wlast);
                 }
             }
-            cnt = static_cast<unsigned>(w - col) / kListBlock;
-            if (cnt > kListCap) fast = false;
+            if (staged) wave_sync();
         }
+        cnt = static_cast<unsigned>(w - col) / kListBlock;
+        if (cnt > kSolidListCap) fast = false;
+    }
+    float rho;
+    if (inear) {
+        rho = density_scan<KAHAN, true>(p, f, i, xi, inear, &nn, nullptr, 0, nullptr);
+        cnt = kListCap + 1u;
+    } else {
         Acc<KAHAN> acc;
         if (fast) {
             for (unsigned k0 = 0; k0 < cnt; k0 += kPairGroup) {
