@@ -409,17 +409,18 @@ __device__ __forceinline__ void solid_scan_round(const V3 (&xs)[kSolidScanUnroll
     }
 }
 // The solid scan's LDS: list columns of kSolidListCap entries and per wave
-// the current and reference positions of one row of up to kSolidStageCap
-// slots (as for the fluid scan).
-constexpr unsigned kSolidListCap = kListCap;
-constexpr unsigned kSolidStageCap = 80;
+// the current positions of one row of up to kSolidStageCap slots (as for the
+// fluid scan).  The reference positions are read from global memory: staging
+// them too costs more occupancy than their reads do.
+constexpr unsigned kSolidListCap = 40;
+constexpr unsigned kSolidStageCap = 128;
+static_assert(kSolidListCap <= kListCap, "a solid list fits the pair cache's columns");
 
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, FieldPtrs f, PairCache c) {
     __shared__ unsigned short buf[kSolidListCap * kListBlock];
-    // per wave: one staged row (current, reference), with room for a round
-    // read past its end
-    __shared__ Packed3 stage[kListBlock / kWave][2][kSolidStageCap + kSolidScanUnroll];
+    // per wave: one staged row, with room for a round read past its end
+    __shared__ Packed3 stage[kListBlock / kWave][kSolidStageCap + kSolidScanUnroll];
     const unsigned i = ps.slot_xcd(kListBlock);
     if (!ps.active(i)) return;
     const StepParams &p = ps.at(i);
@@ -433,8 +434,7 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned key = f.gridcell[i];
     unsigned short *col = buf + threadIdx.x;
-    Packed3 *const stg = stage[threadIdx.x / kWave][0];
-    Packed3 *const stg0 = stage[threadIdx.x / kWave][1];
+    Packed3 *const stg = stage[threadIdx.x / kWave];
     const V3 x0i = load3(opos, i);
     const bool key_in_grid = key >= g.c0 && key - g.c0 < g.ncells;
     bool fast = !inear && key_in_grid;
@@ -470,9 +470,8 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
             const bool staged = se - sb <= kSolidStageCap;
             if (staged) {
                 for (unsigned k = copier; k < se - sb; k += copiers) {
-                    const V3 x = load3(pos, sb + k), x0 = load3(opos, sb + k);
+                    const V3 x = load3(pos, sb + k);
                     stg[k] = Packed3{x.x, x.y, x.z};
-                    stg0[k] = Packed3{x0.x, x0.y, x0.z};
                 }
                 wave_sync();
             }
@@ -492,26 +491,18 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                 V3 xs[kSolidScanUnroll], x0s[kSolidScanUnroll];
                 for (; j0 + kSolidScanUnroll <= e; j0 += kSolidScanUnroll) {
                     const unsigned ent = ebase + (j0 - b);
-                    if (staged) {
-                        candidates_from(stg + (j0 - sb), xs);
-                        candidates_from(stg0 + (j0 - sb), x0s);
-                    } else {
-                        candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
-                        candidates_from(reinterpret_cast<const Packed3 *>(opos) + j0, x0s);
-                    }
+                    if (staged) candidates_from(stg + (j0 - sb), xs);
+                    else candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
+                    candidates_from(reinterpret_cast<const Packed3 *>(opos) + j0, x0s);
                     if (__all(w <= wroom))
                         solid_scan_round<false, false>(xs, x0s, 0u, xi, x0i, r2_out, ent, w, wlast);
                     else solid_scan_round<false, true>(xs, x0s, 0u, xi, x0i, r2_out, ent, w, wlast);
                 }
                 if (j0 < e) {
                     const unsigned ent = ebase + (j0 - b);
-                    if (staged) {
-                        candidates_from(stg + (j0 - sb), xs);
-                        candidates_from(stg0 + (j0 - sb), x0s);
-                    } else {
-                        candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
-                        candidates_from(reinterpret_cast<const Packed3 *>(opos) + j0, x0s);
-                    }
+                    if (staged) candidates_from(stg + (j0 - sb), xs);
+                    else candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
+                    candidates_from(reinterpret_cast<const Packed3 *>(opos) + j0, x0s);
                     if (__all(w <= wroom))
                         solid_scan_round<true, false>(xs, x0s, e - j0, xi, x0i, r2_out, ent, w, wlast);
                     else solid_scan_round<true, true>(xs, x0s, e - j0, xi, x0i, r2_out, ent, w, wlast);
