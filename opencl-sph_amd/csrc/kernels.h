@@ -139,8 +139,9 @@ hipError_t launch_scatter_words(const void *src, void *dst, const unsigned *map,
 // ---- per-particle passes (kernels_fluid.hip, kernels_solid.hip) ------------
 // Neighbour lists written by the density pass, entries k < count[i] at
 // list[k * stride + i] in the reference's visiting order (self included);
-// count[i] > kListCap means the list overflowed (or was not built for slot
-// i) and readers scan the candidates instead.  A fluid list may also hold
+// count[i] above the list's capacity (kListCap solid entries, kFluidListMax
+// fluid entries) means the list overflowed (or was not built for slot i)
+// and readers scan the candidates instead.  A fluid list may also hold
 // extra entries of slot i itself, which the force pass skips as it skips
 // j == i.
 //   kFluidList: the slots j with length(x_j - x_i) <= h, as 16-bit entries
@@ -150,6 +151,7 @@ hipError_t launch_scatter_words(const void *src, void *dst, const unsigned *map,
 //     union of the current and reference neighbourhoods that the solid passes
 //     iterate; slot = entry & kListSlot.
 constexpr unsigned kListCap = 48;
+constexpr unsigned kFluidListMax = 64;   // 16-bit entries, two per word of a column
 constexpr int kFluidList = 1;
 constexpr int kSolidList = 2;
 constexpr unsigned kListCurrent = 1u << 31;

@@ -128,6 +128,13 @@ __global__ void __launch_bounds__(kBlock) k_density(PS ps, FieldPtrs f) {
 // near a mirror plane, with more than kFluidListCap hits or with a dz group
 // spanning more than 16384 slots take the reference loop instead.
 constexpr unsigned kListBlock = 128;
+// Entries of a lane's LDS list column in the density scans (the rest of a
+// longer list goes straight to the pair cache).
+constexpr unsigned kFluidListCap = 40;
+constexpr unsigned kSolidListCap = 40;
+static_assert(kFluidListCap % kPairGroup == 0 && kSolidListCap % kPairGroup == 0,
+              "the kernel sums take whole groups from the LDS column");
+static_assert(kFluidListCap <= kFluidListMax && kSolidListCap <= kListCap, "the LDS column is a prefix of the list");
 
 // First slot of neighbour row r (0..8, (dz, dy) = (r / 3 - 1, r % 3 - 1))
 // around the cell (xo, yo, zo) of the scene's grid, and one past its last
@@ -147,21 +154,43 @@ __device__ __forceinline__ bool row_range(const GridGeom &g, unsigned lo, unsign
     return true;
 }
 
+// Where a lane's list entries past its LDS column go: straight into its
+// column of the pair cache.  Entry k of a fluid list is the low (k even) or
+// high half of word k / 2; entry k of a solid list is word k (slot | flags).
+struct ListSpill {
+    unsigned *list;
+    size_t stride;
+    unsigned i;
+    __device__ __forceinline__ unsigned short *fluid_entry(unsigned k) const {
+        return reinterpret_cast<unsigned short *>(list + (k / 2u) * stride + i) + (k & 1u);
+    }
+    __device__ __forceinline__ unsigned *solid_entry(unsigned k) const { return list + k * stride + i; }
+};
+
 // One round of the fluid scan: the range prefilter (sqlength_fused against
 // r2_wide) on kScanUnroll candidates xs in slot order, appending ent + u to
 // the lane's LDS column at w for each hit.  CHECK_LEFT: the round reaches
 // past the row's end, and only its first `left` candidates are used.  CLAMP:
-// the column may fill up during the round; past its last entry, that entry is
-// overwritten (the list is then discarded, its count being past kListCap).
+// the column may fill up during the round; entries past it go to the pair
+// cache (up to kFluidListMax; beyond that the count alone grows, and the
+// lane's list is discarded).
 template <bool CHECK_LEFT, bool CLAMP>
 __device__ __forceinline__ void scan_round(const V3 (&xs)[kScanUnroll], unsigned left, V3 xi, float r2_wide,
-                                           unsigned ent, unsigned short *&w, unsigned short *wlast) {
+                                           unsigned ent, unsigned short *&w, const unsigned short *col,
+                                           const ListSpill &spill) {
 #pragma unroll
     for (unsigned u = 0; u < kScanUnroll; ++u) {
         const float s = sqlength_fused(xs[u] - xi);
         const bool hit = CHECK_LEFT ? ((u < left) & !(s >= r2_wide)) : !(s >= r2_wide);
         if (hit) {
-            *(CLAMP ? min(w, wlast) : w) = static_cast<unsigned short>(ent + u);
+            const unsigned short v = static_cast<unsigned short>(ent + u);
+            if (CLAMP) {
+                const unsigned k = static_cast<unsigned>(w - col) / kListBlock;
+                if (k < kFluidListCap) *w = v;
+                else if (k < kFluidListMax) *spill.fluid_entry(k) = v;
+            } else {
+                *w = v;
+            }
             w += kListBlock;
         }
     }
@@ -190,8 +219,6 @@ __device__ __forceinline__ void candidates_from(const Packed3 *q, V3 (&xs)[N]) {
 // allow.
 constexpr unsigned kWave = 64;
 constexpr unsigned kStageCap = 96;
-constexpr unsigned kFluidListCap = 40;
-static_assert(kFluidListCap <= kListCap, "a fluid list fits the pair cache's columns");
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -230,6 +257,7 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned key = f.gridcell[i];
     unsigned short *col = buf + threadIdx.x;
+    const ListSpill spill{c.list, c.stride, i};
     Packed3 *const stg = stage[threadIdx.x / kWave];
     unsigned cnt = 0;
     const bool key_in_grid = key >= g.c0 && key - g.c0 < g.ncells;
@@ -252,7 +280,6 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
         // next free entry of the lane's column; the column's last entry; the
         // last position with room for a whole round
         unsigned short *w = col;
-        unsigned short *const wlast = col + (kFluidListCap - 1u) * kListBlock;
         unsigned short *const wroom = col + (kFluidListCap - kScanUnroll) * kListBlock;
         unsigned gbase = 0;   // first slot of the current group's first row inside the grid
         bool gset = false;
@@ -299,22 +326,22 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
                     const unsigned ent = ebase + (j0 - b);
                     if (staged) candidates_from(stg + (j0 - sb), xs);
                     else candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
-                    if (__all(w <= wroom)) scan_round<false, false>(xs, 0u, xi, r2_wide, ent, w, wlast);
-                    else scan_round<false, true>(xs, 0u, xi, r2_wide, ent, w, wlast);
+                    if (__all(w <= wroom)) scan_round<false, false>(xs, 0u, xi, r2_wide, ent, w, col, spill);
+                    else scan_round<false, true>(xs, 0u, xi, r2_wide, ent, w, col, spill);
                 }
                 if (j0 < e) {
                     const unsigned ent = ebase + (j0 - b);
                     if (staged) candidates_from(stg + (j0 - sb), xs);
                     else candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
-                    if (__all(w <= wroom)) scan_round<true, false>(xs, e - j0, xi, r2_wide, ent, w, wlast);
-                    else scan_round<true, true>(xs, e - j0, xi, r2_wide, ent, w, wlast);
+                    if (__all(w <= wroom)) scan_round<true, false>(xs, e - j0, xi, r2_wide, ent, w, col, spill);
+                    else scan_round<true, true>(xs, e - j0, xi, r2_wide, ent, w, col, spill);
                 }
             }
             // this row's reads before the next row's copy
             if (staged) wave_sync();
         }
         cnt = static_cast<unsigned>(w - col) / kListBlock;
-        if (cnt > kFluidListCap) fast = false;
+        if (cnt > kFluidListMax) fast = false;
     }
     float rho;
     unsigned nn = 0;
@@ -328,15 +355,21 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
         const unsigned self_ent = (1u << kGroupBits) | (i - gb.b1);
         for (unsigned k0 = 0; k0 < cnt; k0 += kPairGroup) {
             unsigned es[kPairGroup], js[kPairGroup];
+            // entries of the LDS column, then (k0 is the same for every
+            // lane) those the scan put in the pair cache
 #pragma unroll
-            for (unsigned q = 0; q < kPairGroup; ++q) {
-                js[q] = i;
-                es[q] = 0;
-                if (k0 + q < cnt) {
-                    es[q] = col[(k0 + q) * kListBlock];
-                    js[q] = gb.slot(es[q]);
-                }
+            for (unsigned q = 0; q < kPairGroup; ++q) es[q] = 0;
+            if (k0 < kFluidListCap) {
+#pragma unroll
+                for (unsigned q = 0; q < kPairGroup; ++q)
+                    if (k0 + q < cnt) es[q] = col[(k0 + q) * kListBlock];
+            } else {
+#pragma unroll
+                for (unsigned q = 0; q < kPairGroup; ++q)
+                    if (k0 + q < cnt) es[q] = *spill.fluid_entry(k0 + q);
             }
+#pragma unroll
+            for (unsigned q = 0; q < kPairGroup; ++q) js[q] = k0 + q < cnt ? gb.slot(es[q]) : i;
             V3 xs[kPairGroup];
 #pragma unroll
             for (unsigned q = 0; q < kPairGroup; ++q) xs[q] = load3(pos, js[q]);
@@ -362,7 +395,7 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
     } else {
         // the force pass scans for these lanes (no list)
         rho = density_scan<KAHAN, true>(p, f, i, xi, inear, &nn, nullptr, 0, nullptr);
-        cnt = kListCap + 1u;
+        cnt = kFluidListMax + 1u;
     }
     const float out = rho * p.mass;
     f.density[i] = out;
@@ -391,19 +424,29 @@ constexpr unsigned kSolidOffsetBits = 12;
 // positions xs, reference positions x0s) tested against the current and the
 // reference position of the lane, appending
 // (ent + u) | current << 13 | reference << 12 for each candidate in either
-// range.  CHECK_LEFT and CLAMP as in scan_round.
+// range.  CHECK_LEFT and CLAMP as in scan_round; an entry past the LDS
+// column goes to the pair cache as (slot0 + u) | flags, up to kListCap.
 template <bool CHECK_LEFT, bool CLAMP>
 __device__ __forceinline__ void solid_scan_round(const V3 (&xs)[kSolidScanUnroll], const V3 (&x0s)[kSolidScanUnroll],
                                                  unsigned left, V3 xi, V3 x0i, float r2_out, unsigned ent,
-                                                 unsigned short *&w, unsigned short *wlast) {
+                                                 unsigned slot0, unsigned short *&w, const unsigned short *col,
+                                                 const ListSpill &spill) {
 #pragma unroll
     for (unsigned u = 0; u < kSolidScanUnroll; ++u) {
         const bool cur = !(sqlength(xs[u] - xi) >= r2_out);
         const bool ref = !(sqlength(x0s[u] - x0i) >= r2_out);
         const bool hit = CHECK_LEFT ? ((u < left) & (cur | ref)) : (cur | ref);
         if (hit) {
-            *(CLAMP ? min(w, wlast) : w) =
+            const unsigned short v =
                 static_cast<unsigned short>((ent + u) | (cur ? 1u << 13 : 0u) | (ref ? 1u << 12 : 0u));
+            if (CLAMP) {
+                const unsigned k = static_cast<unsigned>(w - col) / kListBlock;
+                if (k < kSolidListCap) *w = v;
+                else if (k < kListCap)
+                    *spill.solid_entry(k) = (slot0 + u) | (cur ? kListCurrent : 0u) | (ref ? kListReference : 0u);
+            } else {
+                *w = v;
+            }
             w += kListBlock;
         }
     }
@@ -412,9 +455,7 @@ __device__ __forceinline__ void solid_scan_round(const V3 (&xs)[kSolidScanUnroll
 // the current positions of one row of up to kSolidStageCap slots (as for the
 // fluid scan).  The reference positions are read from global memory: staging
 // them too costs more occupancy than their reads do.
-constexpr unsigned kSolidListCap = 40;
 constexpr unsigned kSolidStageCap = 128;
-static_assert(kSolidListCap <= kListCap, "a solid list fits the pair cache's columns");
 
 template <class PS, bool KAHAN>
 __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, FieldPtrs f, PairCache c) {
@@ -434,6 +475,7 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
     const unsigned inear = p.nplanes ? planes_near_i(p, xi) : 0u;
     const unsigned key = f.gridcell[i];
     unsigned short *col = buf + threadIdx.x;
+    const ListSpill spill{c.list, c.stride, i};
     Packed3 *const stg = stage[threadIdx.x / kWave];
     const V3 x0i = load3(opos, i);
     const bool key_in_grid = key >= g.c0 && key - g.c0 < g.ncells;
@@ -453,7 +495,6 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
         const unsigned yo = rem / g.rx;
         const unsigned xo = rem - g.rx * yo;
         unsigned short *w = col;   // as in k_density_listed
-        unsigned short *const wlast = col + (kSolidListCap - 1u) * kListBlock;
         unsigned short *const wroom = col + (kSolidListCap - kSolidScanUnroll) * kListBlock;
         unsigned gbase = 0;
         bool gset = false;
@@ -495,8 +536,8 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                     else candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
                     candidates_from(reinterpret_cast<const Packed3 *>(opos) + j0, x0s);
                     if (__all(w <= wroom))
-                        solid_scan_round<false, false>(xs, x0s, 0u, xi, x0i, r2_out, ent, w, wlast);
-                    else solid_scan_round<false, true>(xs, x0s, 0u, xi, x0i, r2_out, ent, w, wlast);
+                        solid_scan_round<false, false>(xs, x0s, 0u, xi, x0i, r2_out, ent, j0, w, col, spill);
+                    else solid_scan_round<false, true>(xs, x0s, 0u, xi, x0i, r2_out, ent, j0, w, col, spill);
                 }
                 if (j0 < e) {
                     const unsigned ent = ebase + (j0 - b);
@@ -504,14 +545,14 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
                     else candidates_from(reinterpret_cast<const Packed3 *>(pos) + j0, xs);
                     candidates_from(reinterpret_cast<const Packed3 *>(opos) + j0, x0s);
                     if (__all(w <= wroom))
-                        solid_scan_round<true, false>(xs, x0s, e - j0, xi, x0i, r2_out, ent, w, wlast);
-                    else solid_scan_round<true, true>(xs, x0s, e - j0, xi, x0i, r2_out, ent, w, wlast);
+                        solid_scan_round<true, false>(xs, x0s, e - j0, xi, x0i, r2_out, ent, j0, w, col, spill);
+                    else solid_scan_round<true, true>(xs, x0s, e - j0, xi, x0i, r2_out, ent, j0, w, col, spill);
                 }
             }
             if (staged) wave_sync();
         }
         cnt = static_cast<unsigned>(w - col) / kListBlock;
-        if (cnt > kSolidListCap) fast = false;
+        if (cnt > kListCap) fast = false;
     }
     float rho;
     if (inear) {
@@ -521,32 +562,40 @@ __global__ void __launch_bounds__(kListBlock) k_density_solid_listed(PS ps, Fiel
         Acc<KAHAN> acc;
         if (fast) {
             for (unsigned k0 = 0; k0 < cnt; k0 += kPairGroup) {
-                unsigned ents[kPairGroup], js[kPairGroup];
+                // list entries as slot | flags: from the LDS column, then (k0
+                // is the same for every lane) those the scan put in the pair
+                // cache
+                unsigned es[kPairGroup];
 #pragma unroll
-                for (unsigned q = 0; q < kPairGroup; ++q) {
-                    ents[q] = 0;
-                    js[q] = i;
-                    if (k0 + q < cnt) {
-                        ents[q] = col[(k0 + q) * kListBlock];
-                        const unsigned gq = ents[q] >> kGroupBits;
-                        js[q] = (gq == 0u ? b0 : (gq == 1u ? b1 : b2)) +
-                                (ents[q] & ((1u << kSolidOffsetBits) - 1u));
-                    }
+                for (unsigned q = 0; q < kPairGroup; ++q) es[q] = i;
+                if (k0 < kSolidListCap) {
+#pragma unroll
+                    for (unsigned q = 0; q < kPairGroup; ++q)
+                        if (k0 + q < cnt) {
+                            const unsigned ent = col[(k0 + q) * kListBlock];
+                            const unsigned gq = ent >> kGroupBits;
+                            es[q] = ((gq == 0u ? b0 : (gq == 1u ? b1 : b2)) +
+                                     (ent & ((1u << kSolidOffsetBits) - 1u))) |
+                                    (((ent >> 13) & 1u) ? kListCurrent : 0u) |
+                                    (((ent >> 12) & 1u) ? kListReference : 0u);
+                        }
+                } else {
+#pragma unroll
+                    for (unsigned q = 0; q < kPairGroup; ++q)
+                        if (k0 + q < cnt) es[q] = *spill.solid_entry(k0 + q);
                 }
                 V3 xs[kPairGroup];
 #pragma unroll
-                for (unsigned q = 0; q < kPairGroup; ++q) xs[q] = load3(pos, js[q]);
+                for (unsigned q = 0; q < kPairGroup; ++q) xs[q] = load3(pos, es[q] & kListSlot);
 #pragma unroll
                 for (unsigned q = 0; q < kPairGroup; ++q) {
                     if (k0 + q >= cnt) break;
-                    const bool cur = (ents[q] >> 13) & 1u;
-                    if (cur) {
+                    if (es[q] & kListCurrent) {
                         const float rr = length(xs[q] - xi);
                         acc.add(w_poly6(rr, kc));
                         if (rr < kc.h) ++nn;
                     }
-                    c.list[static_cast<size_t>(k0 + q) * c.stride + i] =
-                        js[q] | (cur ? kListCurrent : 0u) | (((ents[q] >> 12) & 1u) ? kListReference : 0u);
+                    c.list[static_cast<size_t>(k0 + q) * c.stride + i] = es[q];
                 }
             }
         } else {
@@ -689,7 +738,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_fluids_listed(PS ps, FieldPtr
     const float visc = p.viscosity;   // read once: with a scene table p is in global memory
     FluidForce<KAHAN> acc;
     const unsigned cnt = c.count[i];
-    if (cnt <= kListCap) {
+    if (cnt <= kFluidListMax) {
         // List entries other than slot i are in range by construction (same
         // positions, same distance expression), so the pair terms need no
         // range test here.

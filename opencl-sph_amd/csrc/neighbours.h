@@ -81,14 +81,14 @@ __device__ __forceinline__ void for_each_listed(const PairCache &c, unsigned i, 
 // for_each_listed over a list of 16-bit entries (the fluid list: entries
 // 2m and 2m + 1 of slot i in the low and high half of word m * stride + i),
 // each turned into its slot by decode(entry).  G is even.  Words are read
-// without a test (the column holds kListCap words, and at most cnt + G
-// entries, cnt <= kListCap, are fetched); entries at or past cnt become i by
-// a select, so that no branch separates the loads.
+// without a test (the column holds kListCap words; with cnt <= kFluidListMax
+// the entries fetched end before cnt + 2 G - 1, inside it); entries at or
+// past cnt become i by a select, so that no branch separates the loads.
 template <unsigned G, typename Decode, typename Load, typename Use>
 __device__ __forceinline__ void for_each_listed16(const unsigned *__restrict__ list, size_t stride, unsigned i,
                                                   unsigned cnt, Decode &&decode, Load &&load, Use &&use) {
     static_assert(G % 2 == 0, "entries are read in pairs");
-    static_assert((kListCap + G) / 2 <= kListCap, "fetched words stay in the slot's column");
+    static_assert((kFluidListMax + 2 * G - 2) / 2 < kListCap, "fetched words stay in the slot's column");
     if (cnt == 0) return;
     const unsigned *__restrict__ col = list + i;
     auto fetch = [&](unsigned k, unsigned *out) {   // entries k, k + 1 (k even)

@@ -137,15 +137,29 @@ def test_large_dam_break_three_steps_bitwise(hip_device):
 @pytest.mark.parametrize("compensated", [0, 3], ids=["plain", "kahan_all"])
 def test_overflowing_lists_bitwise(hip_device, compensated):
     """Packed at dx = h/2.35, most interior particles have more than 48
-    neighbours within h, more than the density pass's neighbour list holds:
-    those lanes take the candidate scan in the density and force passes
-    while the surface lanes read their lists.  Five steps, bitwise."""
+    neighbours within h: past the 40 entries of the density scan's LDS
+    column, their list entries go straight to the pair cache (up to 64), and
+    the few lanes with more take the candidate scan in the density and force
+    passes, while the surface lanes keep short lists.  Five steps, bitwise."""
     scene = scenes.dam_break(20, 20, 10, 0.04 / 2.35, 0.04, 2e-3, seed=11)
     say(f"fluid dense packing, 5 steps, compensated={compensated}, n={scene.n}")
     got, st = run_pair(scene, 5, compensated=compensated)
     check(got, st)
     nn = st.num_neighbours
     assert (nn > 48).mean() > 0.25 and (nn < 47).mean() > 0.25
+
+
+def test_lists_past_the_lds_column_bitwise(hip_device):
+    """At h = 2.2 dx most interior lists hold 41-48 entries, more than the
+    density scan keeps in LDS: the rest of each list is written to the pair
+    cache during the scan and read back by the kernel sum.  Five steps,
+    Kahan density sums, bitwise."""
+    scene = scenes.dam_break(20, 20, 10, 0.04 / 2.2, 0.04, 2e-3, seed=12)
+    say(f"fluid lists past the LDS column, 5 steps, n={scene.n}")
+    got, st = run_pair(scene, 5, compensated=1)
+    check(got, st)
+    nn = st.num_neighbours
+    assert ((nn > 40) & (nn <= 64)).mean() > 0.25
 
 
 def test_crowded_row_takes_the_scan(hip_device):

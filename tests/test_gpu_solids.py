@@ -17,12 +17,16 @@ from sph import scenes
 pytestmark = pytest.mark.gpu
 
 
-def small_solid(kind, nside=9, stretch=1.01, planes=True, constraints=True, bodies=True, seed=4, h=0.02):
+def small_solid(kind, nside=9, stretch=1.01, planes=True, constraints=True, bodies=True, seed=4, h=0.02,
+                jitter=0.0):
     """A cube strained by 1% along x, with mirror planes through its low x
-    and y faces, a plane constraint on its high-x face and a clamped layer."""
+    and y faces, a plane constraint on its high-x face and a clamped layer.
+    jitter: the reference lattice's points moved by up to that much."""
     dx = 0.01
     pts, g = scenes.lattice(nside, nside, nside, dx)
     rng = np.random.Generator(np.random.PCG64(seed))
+    if jitter:
+        pts = pts + rng.uniform(-jitter, jitter, pts.shape)
     n = len(pts)
     lo, hi = pts.min(0) - 4 * h, pts.max(0) + 4 * h
     gb = [float(lo[0]), float(hi[0]), float(lo[1]), float(hi[1]), float(lo[2]), float(hi[2])]
@@ -153,6 +157,26 @@ def test_overflowing_lists_bitwise(hip_device, kind, compensated):
         compare(k, v, st.f[k].reshape(v.shape))
     nn = st.num_neighbours
     assert (nn > 48).mean() > 0.25 and (nn < 47).mean() > 0.1
+
+
+@pytest.mark.parametrize("kind", ["corotational", "svk"])
+def test_lists_past_the_lds_column_bitwise(hip_device, kind):
+    """h = 2.24 dx on a reference lattice jittered by 0.1 dx: about half the
+    lists (current or reference neighbours) hold 41-48 entries, more than the
+    solid density scan keeps in LDS; the rest of each list goes to the pair
+    cache during the scan, and every solid pass reads the whole list.  A few
+    lists exceed 48 and scan.  Three steps, Kahan density sums, bitwise."""
+    kw = dict(nside=12, h=0.0224, planes=False, jitter=1e-3)
+    scene, strained, _ = small_solid(kind, **kw)
+    pos0 = scene.fields["position"]
+    h = np.float32(0.0224)
+    near = lambda x: np.linalg.norm(x[:, None].astype(np.float64) - x[None], axis=2) <= h
+    union = (near(pos0) | near(strained)).sum(1)
+    assert ((union > 40) & (union <= 48)).mean() > 0.3 and (union > 48).any()
+    say(f"{kind}: lists past the LDS column, 3 steps, bitwise")
+    got, st = run_both(kind, 3, compensated=1, **kw)
+    for k, v in got.items():
+        compare(k, v, st.f[k].reshape(v.shape))
 
 
 @pytest.mark.parametrize("kind", ["corotational", "svk"])
