@@ -124,9 +124,10 @@ __global__ void __launch_bounds__(kBlock) k_density(PS ps, FieldPtrs f) {
 //      in range (every lane busy), writing the entries out as the pair
 //      cache's neighbour list; a hit out of range is written as an entry of
 //      the slot itself, which the force pass skips as it skips j == i.
-// The summation order is the candidate order, as in the reference.  Lanes
-// near a mirror plane, with more than kFluidListCap hits or with a dz group
-// spanning more than 16384 slots take the reference loop instead.
+// The summation order is the candidate order, as in the reference.  Hits
+// past the LDS column go straight to the pair cache.  Lanes near a mirror
+// plane, with more than kFluidListMax hits or with a dz group spanning more
+// than 16384 slots take the reference loop instead.
 constexpr unsigned kListBlock = 128;
 // Entries of a lane's LDS list column in the density scans (the rest of a
 // longer list goes straight to the pair cache).
@@ -134,7 +135,8 @@ constexpr unsigned kFluidListCap = 40;
 constexpr unsigned kSolidListCap = 40;
 static_assert(kFluidListCap % kPairGroup == 0 && kSolidListCap % kPairGroup == 0,
               "the kernel sums take whole groups from the LDS column");
-static_assert(kFluidListCap <= kFluidListMax && kSolidListCap <= kListCap, "the LDS column is a prefix of the list");
+static_assert(kFluidListCap <= kFluidListMax && kSolidListCap <= kListCap,
+              "the LDS column is a prefix of the list");
 
 // First slot of neighbour row r (0..8, (dz, dy) = (r / 3 - 1, r % 3 - 1))
 // around the cell (xo, yo, zo) of the scene's grid, and one past its last
@@ -413,11 +415,12 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
 // candidate order.  Two phases per lane as in k_density_listed: the scan
 // compacts candidates in either range into LDS as 16-bit entries (dz group
 // << 14 | current << 13 | reference << 12 | offset from the group's base),
-// then the kernel sum runs over the compacted entries.  The range tests are
-// the square-root-free form of length(d) <= h.  Lanes near a mirror plane
-// take the reference loop and mark their list as overflowed; lanes with more
-// than kListCap entries or a group spanning more than 4096 slots scan again
-// without LDS.
+// then the kernel sum runs over the compacted entries (those past the LDS
+// column go straight to the pair cache as slot | flags).  The range tests
+// are the square-root-free form of length(d) <= h.  Lanes near a mirror
+// plane take the reference loop and mark their list as overflowed; lanes
+// with more than kListCap entries or a group spanning more than 4096 slots
+// scan again without LDS.
 constexpr unsigned kSolidOffsetBits = 12;
 
 // One round of the solid scan: kSolidScanUnroll candidates (current
