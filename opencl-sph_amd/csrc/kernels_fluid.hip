@@ -279,8 +279,8 @@ __global__ void __launch_bounds__(kListBlock) k_density_listed(PS ps, FieldPtrs 
         const unsigned rem = local - g.layer * zo;
         const unsigned yo = rem / g.rx;
         const unsigned xo = rem - g.rx * yo;
-        // next free entry of the lane's column; the column's last entry; the
-        // last position with room for a whole round
+        // next free entry of the lane's column; the last position with room
+        // for a whole round
         unsigned short *w = col;
         unsigned short *const wroom = col + (kFluidListCap - kScanUnroll) * kListBlock;
         unsigned gbase = 0;   // first slot of the current group's first row inside the grid
